@@ -1,0 +1,231 @@
+/*
+ * firesim.h - C ABI of the MI355X (gfx950) fire/smoke step.
+ *
+ * The reference exposes the step as C++ free functions over one process-wide
+ * singleton (src/fire_cuda.h:103-119). This ABI exposes the same operations
+ * over explicit per-scene handles so that many scenes can live on one GPU:
+ *
+ *   reference (src/fire_cuda.h)          this ABI
+ *   fireCudaInitialize      :103   ->    fs_create (+ fs_set_params)
+ *   fireCudaReset           :116   ->    fs_reset
+ *   fireCudaStepD3D11       :110   ->    fs_step            (step only, no render)
+ *   fireCudaStepAndRender   :104   ->    fs_step + fs_render_bgra8
+ *   fireCudaStepAndRenderMeasured :105 -> fs_step + fs_render_bgra8 + fs_metrics
+ *   fireCudaRenderD3D11     :113   ->    fs_render_hdr      (host float4 buffer, no D3D11)
+ *   fireCudaSynchronize     :115   ->    fs_synchronize
+ *   fireCudaShutdown        :118   ->    fs_destroy
+ *   fireCudaLastError       :119   ->    fs_last_error
+ *   FireSettings       :65-101     ->    fs_params (+ fs_emitter list)
+ *   FireCudaFrameMetrics :16-63    ->    fs_metrics
+ *
+ * Conventions: every entry point returns an int status (FS_OK == 0) and never
+ * throws; host buffers belong to the caller; device memory belongs to the
+ * handle. Different handles may be driven from different threads; one handle
+ * must be driven by one thread at a time. All work of a handle runs in order
+ * on that handle's own HIP stream.
+ */
+#ifndef FIRESIM_H
+#define FIRESIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_ABI_VERSION 1
+
+enum fs_status {
+    FS_OK = 0,
+    FS_ERR_INVALID_ARGUMENT = 1,
+    FS_ERR_NOT_INITIALIZED = 2,
+    FS_ERR_HIP = 3,
+    FS_ERR_OUT_OF_MEMORY = 4,
+    FS_ERR_UNSUPPORTED = 5,
+    FS_ERR_IO = 6
+};
+
+/* Scalar, face and light fields (reference globals src/fire_cuda.cu:116-151).
+ * Scalars are nx*ny*nz, x fastest: idx = (z*ny + y)*nx + x (src/fire_cuda.cu:449).
+ * U is (nx+1)*ny*nz, V is nx*(ny+1)*nz, W is nx*ny*(nz+1) (src/fire_cuda.cu:456-487).
+ * LIGHT is 4 floats (rgb radiance, extinction) per light cell, SHADOW 1 float. */
+enum fs_field_id {
+    FS_FIELD_HEAT = 0,
+    FS_FIELD_FUEL = 1,
+    FS_FIELD_OXYGEN = 2,
+    FS_FIELD_SOOT = 3,
+    FS_FIELD_CHAR = 4,
+    FS_FIELD_ASH = 5,
+    FS_FIELD_PYROLYSIS = 6,
+    FS_FIELD_PROGRESS = 7,
+    FS_FIELD_TURBULENCE = 8,
+    FS_FIELD_SOOT_OPTICS = 9,
+    FS_FIELD_U = 10,
+    FS_FIELD_V = 11,
+    FS_FIELD_W = 12,
+    FS_FIELD_PRESSURE = 13,
+    FS_FIELD_DIVERGENCE = 14,
+    FS_FIELD_LIGHT = 15,
+    FS_FIELD_SHADOW = 16,
+    FS_FIELD_COUNT = 17
+};
+
+/* Passes of one step, in the order fs_step runs them (src/fire_cuda.cu:3196-3318). */
+enum fs_stage {
+    FS_STAGE_ADVECT_VELOCITY = 0, /* K5-K7  advectU/V/WKernel :880-932 */
+    FS_STAGE_ADVECT_REACT = 1,    /* K8     advectReactKernel :934-1126 */
+    FS_STAGE_FORCE = 2,           /* K9-K11 force (gather form), boundary clamp, divergence :1128-1199,:1261-1291 */
+    FS_STAGE_PRESSURE = 3,        /* K12    red-black SOR, sor_iterations sweeps :1201-1227 */
+    FS_STAGE_PROJECT = 4,         /* K13-K14 gradient subtract + wall zeroing :1229-1245,:1293-1317 */
+    FS_STAGE_LIGHT = 5,           /* K17-K18 light build + 2 propagations :1445-1565 */
+    FS_STAGE_COUNT = 6
+};
+
+/* Storage precision of the advected scalar fields. */
+enum fs_storage {
+    FS_STORAGE_FP32 = 0, /* every field FP32 (reference) */
+    FS_STORAGE_BF16 = 1  /* heat, fuel, soot stored BF16, all arithmetic FP32 */
+};
+
+/* Scene ids. 0..2 are the reference's procedural fuel beds
+ * (fuelBedMaterial src/fire_cuda.cu:746-793); 3 has no bed, so only
+ * fs_emitter spheres and the mouse sources inject fuel and heat. */
+enum fs_scene_kind {
+    FS_SCENE_ROOM_TRAY = 0,
+    FS_SCENE_CAMPFIRE = 1,
+    FS_SCENE_GAS_BURNER = 2,
+    FS_SCENE_EMITTERS_ONLY = 3
+};
+
+#define FS_MAX_EMITTERS 16
+
+typedef struct fs_scene fs_scene;
+
+typedef struct fs_scene_desc {
+    int32_t nx, ny, nz;        /* grid cells; each >= 4 */
+    int32_t frame_w, frame_h;  /* render target; each >= 1 */
+    int32_t storage;           /* enum fs_storage */
+    int32_t device;            /* HIP device ordinal */
+} fs_scene_desc;
+
+/* Per-step knobs. Field meaning follows FireSettings (src/fire_cuda.h:65-101);
+ * fs_clamp_params applies the reference clamps (src/fire_cuda.cu:3135-3179)
+ * with the extensions noted on each field. All fields are 4 bytes wide. */
+typedef struct fs_params {
+    float dt;                    /* [0.001, 1/30] */
+    float mouse_x, mouse_y;      /* [0, 1] */
+    int32_t left_down, right_down;
+    int32_t scene_id;            /* enum fs_scene_kind, [0, 3] */
+    float wind_x;                /* [-1, 1] (reference "wind", x only) */
+    float wind_z;                /* [-1, 1] extension: same profile on w */
+    float turbulence;            /* [0, 1.8] (reference floor 0.05 lifted to 0) */
+    float detail;                /* [0.1, 1.5] */
+    float smoke;                 /* [0, 1.6] */
+    float intensity;             /* [0.2, 2.4] */
+    float camera_yaw;
+    float camera_pitch;          /* [-0.75, 0.75] */
+    float camera_distance;       /* [1.5, 6] */
+    int32_t cinematic_mode;      /* 0/1 */
+    int32_t raymarch_steps;      /* [24, 256] (reference cap 120 lifted) */
+    int32_t render_debug_mode;   /* [0, 6] */
+    float exposure;              /* [0.45, 2.4] */
+    float reflection_gain;       /* [0, 2.2] */
+    float smoke_darkness;        /* [0.35, 2.4] */
+    float emitter_center_x;      /* [-1.05, 1.05] */
+    float emitter_center_z;      /* [-0.82, 0.82] */
+    float emitter_height_norm;   /* [0, 0.96] */
+    float emitter_height_band_norm; /* [0.006, 0.2] */
+    float emitter_radius;        /* [0.035, 0.8] */
+    int32_t burner_center_count; /* [0, 4] */
+    float burner_center_x[4];
+    float burner_center_y[4];
+    float burner_center_z[4];
+    float sor_omega;             /* [0.05, 1.95]; reference constant 1.52 (:105) */
+    int32_t sor_iterations;      /* [1, 1000]; reference constant 40 (:104) */
+    uint32_t noise_seed;         /* seed of the integer-hash noise */
+} fs_params;
+
+/* Spherical fuel + heat source in voxel units (cell i has centre i + 0.5).
+ * Inside the sphere, each step adds fuel_rate*dt to fuel and raises heat to
+ * (temperature_k - 293)/335 (the reference's heat <-> kelvin map, :1033). */
+typedef struct fs_emitter {
+    float cx, cy, cz, radius;
+    float fuel_rate;
+    float temperature_k;
+} fs_emitter;
+
+/* Frame metrics (FireCudaFrameMetrics src/fire_cuda.h:16-63), FP64 sums. */
+typedef struct fs_metrics {
+    int32_t frame_index;
+    int32_t grid_x, grid_y, grid_z;
+    int32_t pressure_iterations;
+    int32_t invalid_cells;
+    double time_seconds;
+    double heat_sum, fuel_sum, oxygen_sum, soot_sum, char_sum, ash_sum;
+    double pyrolysis_sum, progress_sum, turbulence_energy_sum, soot_optical_depth_sum;
+    double max_heat, max_fuel, max_soot, max_pyrolysis, max_progress, max_turbulence_energy;
+    double flame_height_meters;
+    double mean_optical_depth, mean_scene_light, mean_scene_shadow;
+    double flame_mass_proxy, smoke_mass_proxy, flame_smoke_overlap_proxy;
+    double mean_volumetric_shadow, mean_room_irradiance, heat_release_proxy;
+    double divergence_before_l2, divergence_before_max;
+    double divergence_after_l2, divergence_after_max, divergence_reduction;
+    /* Not in the reference: reaction heat release 7*burnMass summed over cells
+     * (last step and since reset), and the FP64 SOR residual L2. */
+    double heat_release_step;
+    double heat_release_total;
+    double sor_residual_l2;
+} fs_metrics;
+
+/* ---- host-only helpers (no device needed) ---- */
+int fs_abi_version(void);
+void fs_default_params(fs_params* out);
+/* Applies the clamps documented on fs_params; in and out may alias. */
+int fs_clamp_params(const fs_params* in, fs_params* out);
+/* Light grid of a scene: clamp((n+3)/4, lo, hi) (src/fire_cuda.cu:2920-2922). */
+int fs_light_dims(int32_t nx, int32_t ny, int32_t nz, int32_t* lnx, int32_t* lny, int32_t* lnz);
+/* Device bytes fs_create will allocate for desc (admission by HBM budget). */
+int fs_scene_bytes(const fs_scene_desc* desc, uint64_t* bytes);
+/* 32-bit top-down BGRA BMP (reference writeBmp src/main.cpp:2511-2536). */
+int fs_write_bmp(const char* path, const uint32_t* bgra, int32_t width, int32_t height);
+/* Thread-local message of the last failed call that had no handle. */
+const char* fs_last_global_error(void);
+
+/* ---- scene lifecycle ---- */
+int fs_create(const fs_scene_desc* desc, fs_scene** out);
+int fs_destroy(fs_scene* scene);
+int fs_reset(fs_scene* scene);
+int fs_set_params(fs_scene* scene, const fs_params* params);
+int fs_get_params(const fs_scene* scene, fs_params* out);
+int fs_set_emitters(fs_scene* scene, const fs_emitter* emitters, int32_t count);
+const char* fs_last_error(const fs_scene* scene);
+
+/* ---- stepping (asynchronous on the scene stream) ---- */
+int fs_step(fs_scene* scene, int32_t nsteps);
+/* Runs one pass with the current time, without advancing time. */
+int fs_run_stage(fs_scene* scene, int32_t stage);
+int fs_synchronize(fs_scene* scene);
+int fs_set_time(fs_scene* scene, float time, float render_time);
+int fs_get_time(const fs_scene* scene, float* time, float* render_time, int32_t* frame_index);
+
+/* ---- rendering (synchronous: returns when the host buffer is written) ---- */
+int fs_render_hdr(fs_scene* scene, float* rgba_host);        /* frame_w*frame_h*4 floats */
+int fs_render_bgra8(fs_scene* scene, uint32_t* bgra_host);   /* frame_w*frame_h pixels */
+/* Tonemaps a caller HDR frame with the scene's exposure (K21 alone). */
+int fs_tonemap(fs_scene* scene, const float* rgba_host, uint32_t* bgra_host);
+
+/* ---- field access (synchronous) ---- */
+int fs_field_count(const fs_scene* scene, int32_t field, uint64_t* count);
+int fs_read_field(fs_scene* scene, int32_t field, float* host);
+int fs_write_field(fs_scene* scene, int32_t field, const float* host);
+int fs_metrics_collect(fs_scene* scene, fs_metrics* out);
+
+/* ---- measurement support ---- */
+/* Device stream of the scene (hipStream_t) for event timing by the caller. */
+void* fs_stream(fs_scene* scene);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIRESIM_H */

@@ -1,0 +1,393 @@
+// fs_device.hpp - device-side types and helpers shared by the gfx950 kernels.
+//
+// Everything here is FP32 arithmetic on CDNA4 (64-lane wavefronts). The math
+// follows the reference's device library (src/fire_cuda.cu:219-733); the lattice
+// noise uses the PCG integer hash (Jarzynski & Olano, JCGT 2020) in place of the
+// reference's sin-hash so that results do not depend on a sinf implementation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fs {
+
+enum ScalarId { HEAT = 0, FUEL, OXYGEN, SOOT, CHARF, ASH, PYRO, PROGRESS, TURB, SOPT, NSCALAR };
+
+constexpr int kMaxEmitters = 16;
+constexpr float kAdvectionScale = 0.62f;  // src/fire_cuda.cu:107
+
+// Per-pass parameter block (SimParams src/fire_cuda.cu:16-68), passed by value.
+struct DevParams {
+    int nx, ny, nz;
+    int lnx, lny, lnz;
+    int fw, fh;
+    float dt, time;
+    float mouseX, mouseY;
+    int leftDown, rightDown, sceneId;
+    float windX, windZ, turbulence, detail, smokeGain, intensity;
+    int cinematic, raymarchSteps, debugMode;
+    float exposure, reflectionGain, smokeDarkness;
+    float emitterCenterX, emitterCenterZ, emitterHeightNorm, emitterHeightBandNorm, emitterRadius;
+    int burnerCount;
+    float burnerX[4], burnerY[4], burnerZ[4];
+    float omega;
+    int iterations;
+    uint32_t saltNoise2, saltNoise3, saltJitter, saltDither, saltGrain;
+    float camEye[3], camForward[3], camRight[3], camUp[3];
+    float tanHalfFov, aspect;
+    int emitterCount;
+    float emitter[kMaxEmitters][6];  // cx, cy, cz, r (voxels), fuel rate, heat level
+};
+
+// Scalar field set. When BF16 is true, heat, fuel and soot are stored as
+// bfloat16 (round to nearest even) and widened to FP32 on load.
+template <bool BF16>
+struct Scalars {
+    void* f[NSCALAR];
+
+    static constexpr __host__ __device__ bool narrow(int k) { return BF16 && (k == HEAT || k == FUEL || k == SOOT); }
+
+    template <int K>
+    __device__ __forceinline__ float ld(size_t i) const {
+        if constexpr (narrow(K)) {
+            const uint32_t b = static_cast<uint32_t>(static_cast<const uint16_t*>(f[K])[i]) << 16;
+            return __uint_as_float(b);
+        } else {
+            return static_cast<const float*>(f[K])[i];
+        }
+    }
+    template <int K>
+    __device__ __forceinline__ void st(size_t i, float v) const {
+        if constexpr (narrow(K)) {
+            uint32_t b = __float_as_uint(v);
+            b += 0x7FFFu + ((b >> 16) & 1u);
+            static_cast<uint16_t*>(f[K])[i] = static_cast<uint16_t>(b >> 16);
+        } else {
+            static_cast<float*>(f[K])[i] = v;
+        }
+    }
+};
+
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 add3(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 sub3(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 mul3(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ F3 mul3v(F3 a, F3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+__device__ __forceinline__ float sat(float v) { return fminf(1.0f, fmaxf(0.0f, v)); }
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return a + (b - a) * t; }
+__device__ __forceinline__ F3 lerp3(F3 a, F3 b, float t) { return add3(a, mul3(sub3(b, a), t)); }
+__device__ __forceinline__ float lum3(F3 c) { return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f; }
+__device__ __forceinline__ F3 normalize3(F3 v) {
+    const float len = sqrtf(fmaxf(0.000001f, dot3(v, v)));
+    return f3(v.x / len, v.y / len, v.z / len);
+}
+
+// Smoothstep with inverted edges (src/fire_cuda.cu:227-234).
+__device__ __forceinline__ float sstep(float e0, float e1, float x) {
+    const bool inv = e1 < e0;
+    const float lo = inv ? e1 : e0;
+    const float hi = inv ? e0 : e1;
+    const float t = sat((x - lo) / fmaxf(0.000001f, hi - lo));
+    const float v = t * t * (3.0f - 2.0f * t);
+    return inv ? 1.0f - v : v;
+}
+
+// ------------------------------------------------------------- noise ----
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+__device__ __forceinline__ float unit24(uint32_t h) { return static_cast<float>(h >> 8) * (1.0f / 16777216.0f); }
+__device__ __forceinline__ float hash2(int x, int y, uint32_t salt) {
+    return unit24(pcg(pcg(salt + static_cast<uint32_t>(y)) + static_cast<uint32_t>(x)));
+}
+__device__ __forceinline__ float hash3(int x, int y, int z, uint32_t salt) {
+    return unit24(pcg(pcg(pcg(salt + static_cast<uint32_t>(z)) + static_cast<uint32_t>(y)) + static_cast<uint32_t>(x)));
+}
+
+// Value noise, reference interpolation src/fire_cuda.cu:390-399.
+__device__ inline float vnoise2(float px, float py, uint32_t salt) {
+    const float fx0 = floorf(px), fy0 = floorf(py);
+    const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
+    const float fx = px - fx0, fy = py - fy0;
+    // one row hash per lattice y, shared by both x corners
+    const uint32_t r0 = pcg(salt + static_cast<uint32_t>(iy));
+    const uint32_t r1 = pcg(salt + static_cast<uint32_t>(iy + 1));
+    const float a = unit24(pcg(r0 + static_cast<uint32_t>(ix)));
+    const float b = unit24(pcg(r0 + static_cast<uint32_t>(ix + 1)));
+    const float c = unit24(pcg(r1 + static_cast<uint32_t>(ix)));
+    const float d = unit24(pcg(r1 + static_cast<uint32_t>(ix + 1)));
+    const float ux = fx * fx * (3.0f - 2.0f * fx);
+    const float uy = fy * fy * (3.0f - 2.0f * fy);
+    return lerpf(lerpf(a, b, ux), lerpf(c, d, ux), uy);
+}
+
+// src/fire_cuda.cu:401-421
+__device__ inline float vnoise3(float px, float py, float pz, uint32_t salt) {
+    const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
+    const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0), iz = static_cast<int>(fz0);
+    const float fx = px - fx0, fy = py - fy0, fz = pz - fz0;
+    const float ux = fx * fx * (3.0f - 2.0f * fx);
+    const float uy = fy * fy * (3.0f - 2.0f * fy);
+    const float uz = fz * fz * (3.0f - 2.0f * fz);
+    const uint32_t z0 = pcg(salt + static_cast<uint32_t>(iz));
+    const uint32_t z1 = pcg(salt + static_cast<uint32_t>(iz + 1));
+    const uint32_t r00 = pcg(z0 + static_cast<uint32_t>(iy));
+    const uint32_t r10 = pcg(z0 + static_cast<uint32_t>(iy + 1));
+    const uint32_t r01 = pcg(z1 + static_cast<uint32_t>(iy));
+    const uint32_t r11 = pcg(z1 + static_cast<uint32_t>(iy + 1));
+    const uint32_t xa = static_cast<uint32_t>(ix), xb = static_cast<uint32_t>(ix + 1);
+    const float c000 = unit24(pcg(r00 + xa)), c100 = unit24(pcg(r00 + xb));
+    const float c010 = unit24(pcg(r10 + xa)), c110 = unit24(pcg(r10 + xb));
+    const float c001 = unit24(pcg(r01 + xa)), c101 = unit24(pcg(r01 + xb));
+    const float c011 = unit24(pcg(r11 + xa)), c111 = unit24(pcg(r11 + xb));
+    const float c00 = lerpf(c000, c100, ux);
+    const float c10 = lerpf(c010, c110, ux);
+    const float c01 = lerpf(c001, c101, ux);
+    const float c11 = lerpf(c011, c111, ux);
+    return lerpf(lerpf(c00, c10, uy), lerpf(c01, c11, uy), uz);
+}
+
+// 5-octave fbm, src/fire_cuda.cu:423-432
+__device__ inline float fbm2(float px, float py, uint32_t salt) {
+    float sum = 0.0f, amp = 0.5f;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        sum += amp * vnoise2(px, py, salt);
+        px = px * 2.03f + 17.7f;
+        py = py * 2.01f - 9.2f;
+        amp *= 0.5f;
+    }
+    return sum;
+}
+
+// 3-octave fbm, saturated, src/fire_cuda.cu:434-443
+__device__ inline float fbm3(float px, float py, float pz, uint32_t salt) {
+    float sum = 0.0f, amp = 0.5f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        sum += amp * vnoise3(px, py, pz, salt);
+        px = px * 2.03f + 17.7f;
+        py = py * 2.01f - 9.2f;
+        pz = pz * 2.07f + 5.1f;
+        amp *= 0.5f;
+    }
+    return sat(sum * 1.142857f);
+}
+
+// ----------------------------------------------------------- samplers ----
+// Clamped trilinear sample of an x-fastest array of size dx*dy*dz with
+// coordinate multipliers sx, sy, sz (src/fire_cuda.cu:489-600).
+struct Tri {
+    size_t i000, i100, i010, i110, i001, i101, i011, i111;
+    float tx, ty, tz;
+};
+
+__device__ __forceinline__ Tri tri_setup(int dx, int dy, int dz, float sx, float sy, float sz, float u, float v, float w) {
+    u = sat(u);
+    v = sat(v);
+    w = sat(w);
+    const float fx = u * sx, fy = v * sy, fz = w * sz;
+    const int x0 = static_cast<int>(floorf(fx));
+    const int y0 = static_cast<int>(floorf(fy));
+    const int z0 = static_cast<int>(floorf(fz));
+    const int x1 = min(x0 + 1, dx - 1);
+    const int y1 = min(y0 + 1, dy - 1);
+    const int z1 = min(z0 + 1, dz - 1);
+    Tri t;
+    t.tx = fx - static_cast<float>(x0);
+    t.ty = fy - static_cast<float>(y0);
+    t.tz = fz - static_cast<float>(z0);
+    const size_t r00 = (static_cast<size_t>(z0) * dy + y0) * dx;
+    const size_t r10 = (static_cast<size_t>(z0) * dy + y1) * dx;
+    const size_t r01 = (static_cast<size_t>(z1) * dy + y0) * dx;
+    const size_t r11 = (static_cast<size_t>(z1) * dy + y1) * dx;
+    t.i000 = r00 + x0; t.i100 = r00 + x1;
+    t.i010 = r10 + x0; t.i110 = r10 + x1;
+    t.i001 = r01 + x0; t.i101 = r01 + x1;
+    t.i011 = r11 + x0; t.i111 = r11 + x1;
+    return t;
+}
+
+__device__ __forceinline__ float tri_blend(const Tri& t, float c000, float c100, float c010, float c110, float c001, float c101,
+                                           float c011, float c111) {
+    const float c00 = lerpf(c000, c100, t.tx);
+    const float c10 = lerpf(c010, c110, t.tx);
+    const float c01 = lerpf(c001, c101, t.tx);
+    const float c11 = lerpf(c011, c111, t.tx);
+    return lerpf(lerpf(c00, c10, t.ty), lerpf(c01, c11, t.ty), t.tz);
+}
+
+__device__ __forceinline__ float tri_plain(const float* __restrict__ f, const Tri& t) {
+    return tri_blend(t, f[t.i000], f[t.i100], f[t.i010], f[t.i110], f[t.i001], f[t.i101], f[t.i011], f[t.i111]);
+}
+
+template <int K, bool B>
+__device__ __forceinline__ float tri_scalar(const Scalars<B>& S, const Tri& t) {
+    return tri_blend(t, S.template ld<K>(t.i000), S.template ld<K>(t.i100), S.template ld<K>(t.i010), S.template ld<K>(t.i110),
+                     S.template ld<K>(t.i001), S.template ld<K>(t.i101), S.template ld<K>(t.i011), S.template ld<K>(t.i111));
+}
+
+__device__ __forceinline__ Tri tri_scalar_setup(const DevParams& p, float u, float v, float w) {
+    return tri_setup(p.nx, p.ny, p.nz, static_cast<float>(p.nx - 1), static_cast<float>(p.ny - 1), static_cast<float>(p.nz - 1), u, v,
+                     w);
+}
+
+template <int K, bool B>
+__device__ __forceinline__ float sample_scalar(const Scalars<B>& S, const DevParams& p, float u, float v, float w) {
+    return tri_scalar<K, B>(S, tri_scalar_setup(p, u, v, w));
+}
+
+__device__ __forceinline__ float sample_u(const float* __restrict__ f, const DevParams& p, float u, float v, float w) {
+    return tri_plain(f, tri_setup(p.nx + 1, p.ny, p.nz, static_cast<float>(p.nx), static_cast<float>(p.ny - 1),
+                                  static_cast<float>(p.nz - 1), u, v, w));
+}
+__device__ __forceinline__ float sample_v(const float* __restrict__ f, const DevParams& p, float u, float v, float w) {
+    return tri_plain(f, tri_setup(p.nx, p.ny + 1, p.nz, static_cast<float>(p.nx - 1), static_cast<float>(p.ny),
+                                  static_cast<float>(p.nz - 1), u, v, w));
+}
+__device__ __forceinline__ float sample_w(const float* __restrict__ f, const DevParams& p, float u, float v, float w) {
+    return tri_plain(f, tri_setup(p.nx, p.ny, p.nz + 1, static_cast<float>(p.nx - 1), static_cast<float>(p.ny - 1),
+                                  static_cast<float>(p.nz), u, v, w));
+}
+__device__ __forceinline__ F3 sample_vel(const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                         const DevParams& p, float u, float v, float w) {
+    return f3(sample_u(U, p, u, v, w), sample_v(V, p, u, v, w), sample_w(W, p, u, v, w));
+}
+
+// Light grid: float4 trilinear and nearest shadow (src/fire_cuda.cu:618-664).
+__device__ __forceinline__ float4 sample_light(const float4* __restrict__ L, const DevParams& p, float u, float v, float w) {
+    const Tri t = tri_setup(p.lnx, p.lny, p.lnz, static_cast<float>(p.lnx - 1), static_cast<float>(p.lny - 1),
+                            static_cast<float>(p.lnz - 1), u, v, w);
+    const float4 a = L[t.i000], b = L[t.i100], c = L[t.i010], d = L[t.i110];
+    const float4 e = L[t.i001], f = L[t.i101], g = L[t.i011], h = L[t.i111];
+    float4 o;
+    o.x = tri_blend(t, a.x, b.x, c.x, d.x, e.x, f.x, g.x, h.x);
+    o.y = tri_blend(t, a.y, b.y, c.y, d.y, e.y, f.y, g.y, h.y);
+    o.z = tri_blend(t, a.z, b.z, c.z, d.z, e.z, f.z, g.z, h.z);
+    o.w = tri_blend(t, a.w, b.w, c.w, d.w, e.w, f.w, g.w, h.w);
+    return o;
+}
+__device__ __forceinline__ float sample_shadow(const float* __restrict__ S, const DevParams& p, float u, float v, float w) {
+    u = sat(u);
+    v = sat(v);
+    w = sat(w);
+    const int x = min(max(static_cast<int>(u * static_cast<float>(p.lnx - 1) + 0.5f), 0), p.lnx - 1);
+    const int y = min(max(static_cast<int>(v * static_cast<float>(p.lny - 1) + 0.5f), 0), p.lny - 1);
+    const int z = min(max(static_cast<int>(w * static_cast<float>(p.lnz - 1) + 0.5f), 0), p.lnz - 1);
+    return S[(static_cast<size_t>(z) * p.lny + y) * p.lnx + x];
+}
+
+// --------------------------------------------------------- fuel beds ----
+// src/fire_cuda.cu:734-744
+__device__ inline float capsule_mask(float x, float z, float cx, float cz, float angle, float halfLength, float radius) {
+    float sa, ca;
+    sincosf(angle, &sa, &ca);
+    const float px = x - cx, pz = z - cz;
+    const float qx = px * ca + pz * sa;
+    const float qz = -px * sa + pz * ca;
+    const float over = fmaxf(fabsf(qx) - halfLength, 0.0f);
+    return sstep(radius, radius * 0.22f, sqrtf(over * over + qz * qz));
+}
+
+// Procedural bed material (src/fire_cuda.cu:746-793); scene 3 has none.
+__device__ inline float bed_material(float x, float z, const DevParams& p) {
+    if (p.sceneId == 3) return 0.0f;
+    const uint32_t s2 = p.saltNoise2;
+    if (p.sceneId == 1) {
+        const float logA = capsule_mask(x, z, -0.06f, 0.01f, 0.38f, 0.34f, 0.060f);
+        const float logB = capsule_mask(x, z, 0.07f, -0.02f, -0.58f, 0.33f, 0.058f);
+        const float logC = capsule_mask(x, z, 0.01f, 0.08f, 1.34f, 0.24f, 0.050f);
+        const float cAB = fminf(logA, logB) * 1.25f;
+        const float cBC = fminf(logB, logC) * 1.10f;
+        const float cCA = fminf(logC, logA) * 0.92f;
+        const float coalNoise = fbm2(x * 9.4f + z * 1.7f, z * 11.2f - x * 3.1f, s2);
+        const float coalBed = sstep(0.38f, 0.08f, sqrtf(x * x * 1.35f + z * z * 2.40f)) * sstep(0.34f, 0.88f, coalNoise);
+        const float cracks = sstep(0.65f, 0.93f, fbm2(x * 21.0f - z * 6.0f, z * 24.0f + x * 4.0f, s2));
+        const float contact = sat(cAB + cBC + cCA);
+        return sat((logA * 0.72f + logB * 0.70f + logC * 0.54f + coalBed * 0.84f + contact * 1.18f) * (1.0f - cracks * 0.24f));
+    }
+    if (p.sceneId == 2) {
+        const float R = fmaxf(0.04f, p.emitterRadius);
+        const float cx = p.burnerCount > 0 ? p.burnerX[0] : p.emitterCenterX;
+        const float cz = p.burnerCount > 0 ? p.burnerZ[0] : p.emitterCenterZ;
+        const float px = x - cx, pz = z - cz;
+        const float radius = sqrtf(px * px + pz * pz);
+        const float angle = atan2f(pz, px);
+        const float ring = sstep(R * 0.072f, R * 0.014f, fabsf(radius - R));
+        const float inner = sstep(R * 0.052f, R * 0.012f, fabsf(radius - R * 0.52f));
+        const float ports = 0.5f + 0.5f * cosf(angle * 32.0f);
+        const float portMask = sstep(0.80f, 0.992f, ports);
+        const float pilot = sstep(R * 0.22f, 0.000f, radius) * 0.045f;
+        const float source = fmaxf(0.0f, (ring * (0.035f + portMask * 1.72f) + inner * 0.055f + pilot) * sstep(R * 1.28f, R * 1.05f, radius));
+        return sat(source);
+    }
+    const float tray = sstep(1.08f, 0.030f, fabsf(x)) * sstep(0.56f, 0.026f, fabsf(z));
+    const float emberMat = sstep(1.00f, 0.018f, fabsf(x)) * sstep(0.50f, 0.020f, fabsf(z));
+    const float mound = expf(-(x * x * 0.70f + z * z * 2.15f));
+    const float logMass = fbm2(x * 3.7f + z * 0.9f, z * 5.4f - x * 0.6f, s2);
+    const float chips = fbm2(x * 15.0f + z * 4.0f, z * 18.0f - x * 5.0f, s2);
+    const float strand = 0.5f + 0.5f * sinf((x * 19.0f + z * 7.0f) + fbm2(z * 4.0f, x * 3.0f, s2) * 4.2f);
+    const float cracks = sstep(0.44f, 0.74f, chips) * sstep(0.20f, 0.84f, logMass);
+    const float voids = sstep(0.70f, 0.93f, chips * 0.72f + strand * 0.34f);
+    const float clumps = sstep(0.22f, 0.86f, logMass * 0.82f + chips * 0.36f + strand * 0.18f);
+    const float porous = (0.24f + clumps * 0.94f + emberMat * 0.20f) * (0.64f + mound * 0.46f);
+    return sat(tray * porous * (1.0f - cracks * 0.46f) * (1.0f - voids * 0.34f));
+}
+
+// src/fire_cuda.cu:795-803, given the material already evaluated.
+__device__ inline float bed_source(float material, float x, float z, float h, const DevParams& p) {
+    if (material == 0.0f) return 0.0f;
+    const float height = p.sceneId == 2
+        ? sstep(fmaxf(0.005f, p.emitterHeightBandNorm * 0.34f), fmaxf(0.0015f, p.emitterHeightBandNorm * 0.055f), fabsf(h - p.emitterHeightNorm))
+        : sstep(p.sceneId == 1 ? 0.245f : 0.145f, 0.00f, h);
+    const float breathing = 0.84f + 0.16f * fbm2(x * 9.0f + p.time * 0.035f, z * 11.0f - p.time * 0.025f, p.saltNoise2);
+    return sat(height * sat(material * breathing));
+}
+
+// src/fire_cuda.cu:1420-1443
+__device__ inline F3 blackbody(float tempK) {
+    const float t = fmaxf(1000.0f, fminf(12000.0f, tempK)) / 100.0f;
+    float r, g, b;
+    if (t <= 66.0f) {
+        r = 1.0f;
+        g = sat((99.4708025861f * logf(t) - 161.1195681661f) / 255.0f);
+    } else {
+        r = sat(329.698727446f * powf(t - 60.0f, -0.1332047592f) / 255.0f);
+        g = sat(288.1221695283f * powf(t - 60.0f, -0.0755148492f) / 255.0f);
+    }
+    if (t >= 66.0f) {
+        b = 1.0f;
+    } else if (t <= 19.0f) {
+        b = 0.0f;
+    } else {
+        b = sat((138.5177312231f * logf(t - 10.0f) - 305.0447927307f) / 255.0f);
+    }
+    return f3(r, g, b);
+}
+
+// src/fire_cuda.cu:359-368
+__device__ __forceinline__ float domain_fade(float u, float v, float w) {
+    const float side = sstep(0.000f, 0.095f, u) * sstep(1.000f, 0.905f, u) * sstep(0.000f, 0.100f, w) * sstep(1.000f, 0.900f, w);
+    return sat(side * sstep(0.000f, 0.018f, v) * sstep(1.000f, 0.520f, v));
+}
+
+// ------------------------------------------------------------ reduce ----
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+}  // namespace fs

@@ -1,0 +1,666 @@
+// step_kernels.hip - simulation passes of one step on gfx950.
+//
+// Passes (reference kernels in src/fire_cuda.cu):
+//   k_advect_velocity   K5-K7  :880-932   one launch for u, v and w faces
+//   k_advect_react      K8     :934-1126  MacCormack x8 + combustion, FP64 heat-release partials
+//   k_force_contrib     K9     :1128-1183 per-cell force contributions (gather form)
+//   k_force_apply       K9-K11 :1128-1199,:1261-1291 faces += contributions, clamp, divergence
+//   k_sor_half          K12    :1201-1227 one red/black half-sweep
+//   k_project           K13-14 :1229-1245,:1293-1317 gradient subtract + wall zeroing
+//   k_reset             K1-K4  :805-878
+//   k_metrics           K15-16 :1247-1259,:1319-1418 FP64 block partials + k_reduce_metrics
+#include "fs_launch.hpp"
+
+namespace fs {
+
+dim3 cell_block() { return dim3(64, 4, 1); }
+dim3 cell_grid(int nx, int ny, int nz) {
+    const dim3 b = cell_block();
+    return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y, (nz + b.z - 1) / b.z);
+}
+size_t react_partial_count(int nx, int ny, int nz) {
+    const dim3 g = cell_grid(nx, ny, nz);
+    return static_cast<size_t>(g.x) * g.y * g.z;
+}
+
+__device__ __forceinline__ size_t sidx(const DevParams& p, int x, int y, int z) {
+    return (static_cast<size_t>(z) * p.ny + y) * p.nx + x;
+}
+__device__ __forceinline__ size_t uidx(const DevParams& p, int x, int y, int z) {
+    return (static_cast<size_t>(z) * p.ny + y) * (p.nx + 1) + x;
+}
+__device__ __forceinline__ size_t vidx(const DevParams& p, int x, int y, int z) {
+    return (static_cast<size_t>(z) * (p.ny + 1) + y) * p.nx + x;
+}
+__device__ __forceinline__ size_t widx(const DevParams& p, int x, int y, int z) {
+    return (static_cast<size_t>(z) * p.ny + y) * p.nx + x;
+}
+
+// Sum over the 256-thread block in FP64; thread 0 returns the total.
+__device__ __forceinline__ double block_sum_256(double v) {
+    __shared__ double lds[4];
+    const int lane = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    v = wave_sum(v);
+    if ((lane & 63) == 0) lds[lane >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (lane == 0) t = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+    return t;
+}
+
+// ------------------------------------------------------------ K5-K7 ----
+__global__ __launch_bounds__(256) void k_advect_velocity(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                                                         const float* __restrict__ U, const float* __restrict__ V,
+                                                         const float* __restrict__ W, DevParams p) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x > p.nx || y > p.ny || z > p.nz) return;
+    const float fnx = static_cast<float>(p.nx), fny = static_cast<float>(p.ny), fnz = static_cast<float>(p.nz);
+    if (y < p.ny && z < p.nz) {
+        float out = 0.0f;
+        if (x != 0 && x != p.nx) {
+            const float fu = static_cast<float>(x) / fnx, fv = (static_cast<float>(y) + 0.5f) / fny, fw = (static_cast<float>(z) + 0.5f) / fnz;
+            const F3 vel = sample_vel(U, V, W, p, fu, fv, fw);
+            out = sample_u(U, p, fu - vel.x * p.dt * kAdvectionScale, fv - vel.y * p.dt * kAdvectionScale, fw - vel.z * p.dt * kAdvectionScale) *
+                  0.995f;
+        }
+        uo[uidx(p, x, y, z)] = out;
+    }
+    if (x < p.nx && z < p.nz) {
+        float out = 0.0f;
+        if (y != 0) {
+            const float fu = (static_cast<float>(x) + 0.5f) / fnx, fv = static_cast<float>(y) / fny, fw = (static_cast<float>(z) + 0.5f) / fnz;
+            const F3 vel = sample_vel(U, V, W, p, fu, fv, fw);
+            out = sample_v(V, p, fu - vel.x * p.dt * kAdvectionScale, fv - vel.y * p.dt * kAdvectionScale, fw - vel.z * p.dt * kAdvectionScale) *
+                  0.996f;
+        }
+        vo[vidx(p, x, y, z)] = out;
+    }
+    if (x < p.nx && y < p.ny) {
+        float out = 0.0f;
+        if (z != 0 && z != p.nz) {
+            const float fu = (static_cast<float>(x) + 0.5f) / fnx, fv = (static_cast<float>(y) + 0.5f) / fny, fw = static_cast<float>(z) / fnz;
+            const F3 vel = sample_vel(U, V, W, p, fu, fv, fw);
+            out = sample_w(W, p, fu - vel.x * p.dt * kAdvectionScale, fv - vel.y * p.dt * kAdvectionScale, fw - vel.z * p.dt * kAdvectionScale) *
+                  0.995f;
+        }
+        wo[widx(p, x, y, z)] = out;
+    }
+}
+
+// --------------------------------------------------------------- K8 ----
+// MacCormack step sharing the backtrace and forward-trace cells of all fields
+// (sampleScalarMacCormack src/fire_cuda.cu:713-732).
+template <int K, bool B>
+__device__ __forceinline__ float maccormack(const Scalars<B>& S, const Tri& tp, const Tri& tf, float current, float blend) {
+    const float c000 = S.template ld<K>(tp.i000), c100 = S.template ld<K>(tp.i100);
+    const float c010 = S.template ld<K>(tp.i010), c110 = S.template ld<K>(tp.i110);
+    const float c001 = S.template ld<K>(tp.i001), c101 = S.template ld<K>(tp.i101);
+    const float c011 = S.template ld<K>(tp.i011), c111 = S.template ld<K>(tp.i111);
+    const float first = tri_blend(tp, c000, c100, c010, c110, c001, c101, c011, c111);
+    const float lo = fminf(fminf(fminf(c000, c100), fminf(c010, c110)), fminf(fminf(c001, c101), fminf(c011, c111)));
+    const float hi = fmaxf(fmaxf(fmaxf(c000, c100), fmaxf(c010, c110)), fmaxf(fmaxf(c001, c101), fmaxf(c011, c111)));
+    const float round = tri_scalar<K, B>(S, tf);
+    const float corrected = fminf(hi, fmaxf(lo, first + 0.5f * (current - round)));
+    return lerpf(first, corrected, sat(blend));
+}
+
+template <bool B>
+__device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const float* __restrict__ U, const float* __restrict__ V,
+                            const float* __restrict__ W, const DevParams& p, int x, int y, int z) {
+    const size_t idx = sidx(p, x, y, z);
+    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
+    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
+    const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const float dt = p.dt;
+    const int sid = p.sceneId;
+    const float h0 = in.template ld<HEAT>(idx), f0 = in.template ld<FUEL>(idx), o0 = in.template ld<OXYGEN>(idx);
+    const float s0 = in.template ld<SOOT>(idx), ch0 = in.template ld<CHARF>(idx), a0 = in.template ld<ASH>(idx);
+    const float py0 = in.template ld<PYRO>(idx), pr0 = in.template ld<PROGRESS>(idx), k0 = in.template ld<TURB>(idx);
+    const float so0 = in.template ld<SOPT>(idx);
+    const float activity = h0 + f0 * 0.18f + s0 * 0.24f + py0 * 0.20f + pr0 * 0.14f + so0 * 0.08f + k0 * 0.06f;
+    const float sdx = u - p.mouseX, sdy = v - p.mouseY, sdz = w - 0.5f;
+    const bool outsideInteractive = p.leftDown == 0 || (sdx * sdx + sdy * sdy + sdz * sdz) > 0.090f;
+    uint32_t inside = 0u;
+    const float cx = static_cast<float>(x) + 0.5f, cy = static_cast<float>(y) + 0.5f, cz = static_cast<float>(z) + 0.5f;
+    for (int e = 0; e < p.emitterCount; ++e) {
+        const float ddx = cx - p.emitter[e][0], ddy = cy - p.emitter[e][1], ddz = cz - p.emitter[e][2];
+        if (ddx * ddx + ddy * ddy + ddz * ddz <= p.emitter[e][3] * p.emitter[e][3]) inside |= 1u << e;
+    }
+    if (v > 0.18f && activity < 0.00035f && ch0 < 0.00010f && a0 < 0.00010f && outsideInteractive && inside == 0u && p.rightDown == 0) {
+        out.template st<HEAT>(idx, 0.0f);
+        out.template st<FUEL>(idx, 0.0f);
+        out.template st<OXYGEN>(idx, fminf(1.0f, o0 + (1.0f - o0) * dt * 0.08f));
+        out.template st<SOOT>(idx, 0.0f);
+        out.template st<CHARF>(idx, ch0);
+        out.template st<ASH>(idx, a0);
+        out.template st<PYRO>(idx, 0.0f);
+        out.template st<PROGRESS>(idx, 0.0f);
+        out.template st<TURB>(idx, k0 * fmaxf(0.0f, 1.0f - dt * 0.54f));
+        out.template st<SOPT>(idx, 0.0f);
+        return 0.0f;
+    }
+    const F3 vel = sample_vel(U, V, W, p, u, v, w);
+    const float pu = u - vel.x * dt * kAdvectionScale, pv = v - vel.y * dt * kAdvectionScale, pw = w - vel.z * dt * kAdvectionScale;
+    const F3 pvel = sample_vel(U, V, W, p, pu, pv, pw);
+    const Tri tp = tri_scalar_setup(p, pu, pv, pw);
+    const Tri tf = tri_scalar_setup(p, pu + pvel.x * dt * kAdvectionScale, pv + pvel.y * dt * kAdvectionScale, pw + pvel.z * dt * kAdvectionScale);
+    const float anti = sat(0.30f + p.detail * 0.34f);
+    float heat = maccormack<HEAT, B>(in, tp, tf, h0, anti);
+    float fuel = maccormack<FUEL, B>(in, tp, tf, f0, anti * 0.92f);
+    float oxygen = maccormack<OXYGEN, B>(in, tp, tf, o0, anti * 0.70f);
+    float soot = maccormack<SOOT, B>(in, tp, tf, s0, anti);
+    float charMass = ch0;
+    float ash = a0;
+    float pyro = maccormack<PYRO, B>(in, tp, tf, py0, anti * 0.72f) * 0.55f;
+    float progress = maccormack<PROGRESS, B>(in, tp, tf, pr0, anti * 0.82f);
+    float turb = maccormack<TURB, B>(in, tp, tf, k0, anti * 0.52f);
+    float sootOptics = maccormack<SOPT, B>(in, tp, tf, so0, anti * 0.72f);
+
+    const float worldX = (u - 0.50f) * 2.10f;
+    const float worldZ = (w - 0.50f) * 1.64f;
+    const float material = bed_material(worldX, worldZ, p);
+    const float exposedChar = sat(charMass / fmaxf(0.055f, charMass + ash * 1.75f));
+    const float charEdges = sstep(0.06f, 0.78f, material) * sstep(0.92f, 0.02f, ash);
+    const float gasFeed = sid == 2 ? 1.18f : exposedChar * charEdges;
+    const float bedSrc = bed_source(material, worldX, worldZ, v, p);
+    // The source noise only matters when a bed is exposed or char can release gas.
+    float sourceNoise = 0.70f;
+    if (bedSrc * gasFeed != 0.0f || charMass != 0.0f) {
+        sourceNoise = 0.70f + 0.48f * fbm2(worldX * 13.0f + worldZ * 3.0f + p.time * 0.38f, v * 31.0f + worldZ * 11.0f, p.saltNoise2);
+    }
+    const float pyroGain = sid == 2 ? 0.42f : (sid == 1 ? 1.34f : 1.0f);
+    const float fuelGain = sid == 2 ? 0.46f : (sid == 1 ? 1.18f : 1.0f);
+    const float heatGain = sid == 2 ? 1.18f : (sid == 1 ? 1.08f : 1.0f);
+    const float sootGain = sid == 2 ? 0.18f : (sid == 1 ? 1.22f : 1.0f);
+    const float bed = bedSrc * sourceNoise * gasFeed * p.intensity;
+    const float ambientK = 293.0f;
+    float tempK = ambientK + heat * 335.0f;
+    const float radFb = sstep(430.0f, 980.0f, tempK);
+    const float solidO2 = sat(oxygen * 1.28f - soot * 0.030f);
+    const float heatFlux = sat((tempK - 405.0f) / 900.0f) + bed * 0.34f + sootOptics * 0.024f;
+    const float charRelease = fminf(charMass, charMass * heatFlux * solidO2 * (0.14f + radFb * 1.38f) * dt);
+    pyro += (bed * (0.34f + radFb * 1.05f) * dt + charRelease * (2.45f + sourceNoise * 0.72f)) * pyroGain;
+    charMass += material * dt * (sid == 2 ? 0.00005f : (sid == 1 ? 0.050f : 0.026f));
+    charMass -= charRelease * 0.92f;
+    ash += charRelease * (sid == 2 ? 0.025f : 0.16f + 0.22f * sat(1.0f - oxygen));
+    heat += (bed * (sid == 2 ? 10.40f : 6.20f) * dt * (1.0f + radFb * 1.12f) + charRelease * 3.35f) * heatGain;
+    fuel += pyro * (sid == 2 ? 1.34f : 3.70f) * fuelGain;
+    soot += pyro * (0.090f + ash * 0.032f + sat(1.0f - oxygen) * 0.13f) * p.smokeGain * (sid == 2 ? 0.010f : (sid == 1 ? 0.92f : 1.0f)) * sootGain;
+
+    if (p.leftDown != 0) {
+        const float mx = p.mouseX, my = p.mouseY;
+        const float source = expf(-((u - mx) * (u - mx)) / 0.0180f - ((w - 0.5f) * (w - 0.5f)) / 0.0550f - ((v - my) * (v - my)) / 0.0020f);
+        heat += source * dt * 2.8f;
+        fuel += source * dt * 1.4f;
+        pyro += source * dt * 0.45f;
+        progress += source * dt * 0.75f;
+        turb += source * dt * 0.28f;
+        oxygen = fmaxf(oxygen, 0.72f + source * 0.20f);
+    }
+    if (p.rightDown != 0) {
+        const float source = expf(-((u - p.mouseX) * (u - p.mouseX) + (w - 0.5f) * (w - 0.5f)) / 0.0100f - (v - p.mouseY) * (v - p.mouseY) / 0.0100f);
+        soot += source * dt * 8.0f;
+        heat *= 1.0f - source * dt * 1.2f;
+        fuel *= 1.0f - source * dt * 0.7f;
+    }
+    for (int e = 0; e < p.emitterCount; ++e) {
+        if (inside & (1u << e)) {
+            fuel += p.emitter[e][4] * dt;
+            heat = fmaxf(heat, p.emitter[e][5]);
+        }
+    }
+
+    tempK = ambientK + heat * 335.0f;
+    const float thermal = sstep(575.0f, 1220.0f, tempK);
+    const float arrhenius = expf(-3800.0f / fmaxf(640.0f, tempK));
+    const float oxyLim = fminf(fuel, oxygen * 1.42f);
+    const float burnMass = fminf(oxyLim, oxyLim * thermal * arrhenius * (118.0f + p.intensity * 34.0f) * dt);
+    const float incomplete = sat(1.0f - oxygen * 1.30f + fuel * 0.065f + soot * 0.015f);
+    const float release = burnMass * 7.0f;
+    const float sootYield = burnMass * (0.035f + incomplete * 0.34f) * p.smokeGain * (sid == 2 ? 0.006f : (sid == 1 ? 0.96f : 1.0f));
+    const float sootOxidation = soot * oxygen * sstep(760.0f, 1540.0f, tempK) * dt * 0.82f;
+    const float front = burnMass * (9.8f + turb * 3.4f) + pyro * 0.70f + charRelease * 2.10f;
+    const float bA = sat(sstep(0.70f, 0.96f, v) + sstep(0.11f, 0.03f, u) + sstep(0.89f, 0.97f, u) + sstep(0.11f, 0.03f, w) + sstep(0.89f, 0.97f, w));
+    const float upper = sstep(0.52f, 0.98f, v);
+
+    heat += release;
+    fuel -= burnMass * 1.20f;
+    oxygen -= burnMass * 0.98f;
+    soot += sootYield;
+    soot -= sootOxidation;
+    progress += front;
+    progress *= expf(-dt * (0.72f + v * 0.45f + oxygen * 0.16f + bA * 1.05f + upper * 0.26f));
+    oxygen = lerpf(oxygen, 1.0f, bA * dt * 1.7f);
+    oxygen += (1.0f - oxygen) * dt * 0.08f;
+
+    tempK = ambientK + heat * 335.0f;
+    const float radBase = fmaxf(0.0f, (tempK - ambientK) / 1560.0f);
+    const float radSq = radBase * radBase;
+    heat -= radSq * radSq * dt * (0.88f + soot * 0.085f + sootOptics * 0.018f);
+    heat *= expf(-dt * (0.31f + v * 0.64f + soot * 0.022f + bA * 0.26f + upper * 0.18f));
+    fuel *= expf(-dt * (0.58f + heat * 0.20f + v * 0.92f + bA * 0.18f));
+    soot *= expf(-dt * (0.060f + v * 0.092f + oxygen * 0.028f + bA * 0.96f + upper * 0.22f));
+    pyro *= expf(-dt * (1.55f + v * 0.58f + bA * 0.18f));
+
+    const float uL = U[uidx(p, x, y, z)], uR = U[uidx(p, x + 1, y, z)];
+    const float vD = V[vidx(p, x, y, z)], vU = V[vidx(p, x, y + 1, z)];
+    const float wB = W[widx(p, x, y, z)], wF = W[widx(p, x, y, z + 1)];
+    const float shear = fabsf(uR - uL) + fabsf(vU - vD) + fabsf(wF - wB);
+    const float flameActivity = sat(progress * 0.64f + burnMass * 5.2f + heat * 0.065f);
+    turb += dt * (shear * (0.72f + progress * 0.06f) + flameActivity * p.turbulence * 0.62f + front * 0.085f + soot * 0.018f);
+    turb *= expf(-dt * (0.54f + v * 0.24f + bA * 0.42f + upper * 0.16f));
+    const float growth = sat(incomplete * 0.36f + ash * 0.13f + soot * 0.055f + sstep(0.65f, 1.55f, tempK / 1000.0f) * 0.18f);
+    sootOptics = soot * (1.04f + 0.28f * sqrtf(fmaxf(0.0f, heat)) + 0.24f * ash + 0.46f * growth);
+    sootOptics = lerpf(sootOptics, sootOptics * 0.48f, sat(sootOxidation * 0.92f));
+
+    out.template st<HEAT>(idx, fminf(8.8f, fmaxf(0.0f, heat)));
+    out.template st<FUEL>(idx, fminf(5.5f, fmaxf(0.0f, fuel)));
+    out.template st<OXYGEN>(idx, fminf(1.0f, fmaxf(0.0f, oxygen)));
+    out.template st<SOOT>(idx, fminf(7.4f, fmaxf(0.0f, soot)));
+    out.template st<CHARF>(idx, fminf(2.8f, fmaxf(0.0f, charMass)));
+    out.template st<ASH>(idx, fminf(1.9f, fmaxf(0.0f, ash)));
+    out.template st<PYRO>(idx, fminf(3.6f, fmaxf(0.0f, pyro)));
+    out.template st<PROGRESS>(idx, fminf(3.4f, fmaxf(0.0f, progress)));
+    out.template st<TURB>(idx, fminf(4.2f, fmaxf(0.0f, turb)));
+    out.template st<SOPT>(idx, fminf(10.0f, fmaxf(0.0f, sootOptics)));
+    return release;
+}
+
+template <bool B>
+__global__ __launch_bounds__(256) void k_advect_react(Scalars<B> out, Scalars<B> in, const float* __restrict__ U, const float* __restrict__ V,
+                                                      const float* __restrict__ W, DevParams p, double* __restrict__ partial) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    float release = 0.0f;
+    if (x < p.nx && y < p.ny && z < p.nz) release = react_cell<B>(out, in, U, V, W, p, x, y, z);
+    const double t = block_sum_256(static_cast<double>(release));
+    if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0)
+        partial[(static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+}
+
+// Sums the react partials in a fixed order: acc[0] = step total, acc[1] += it.
+__global__ __launch_bounds__(256) void k_finalize_release(const double* __restrict__ partial, size_t n, double* __restrict__ acc) {
+    double s = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    const double t = block_sum_256(s);
+    if (threadIdx.x == 0) {
+        acc[0] = t;
+        acc[1] += t;
+    }
+}
+
+// ---------------------------------------------------------- K9-K11 ----
+template <bool B>
+__global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, Scalars<B> S,
+                                                       const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                                       DevParams p) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= p.nx || y >= p.ny || z >= p.nz) return;
+    const size_t idx = sidx(p, x, y, z);
+    const int sid = p.sceneId;
+    const float h = S.template ld<HEAT>(idx), s = S.template ld<SOOT>(idx), k = S.template ld<TURB>(idx), c = S.template ld<PROGRESS>(idx);
+    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
+    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
+    const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const int xm = max(x - 1, 0), xp = min(x + 1, p.nx - 1);
+    const int ym = max(y - 1, 0), yp = min(y + 1, p.ny - 1);
+    const int zm = max(z - 1, 0), zp = min(z + 1, p.nz - 1);
+    const float gradX = S.template ld<HEAT>(sidx(p, xp, y, z)) - S.template ld<HEAT>(sidx(p, xm, y, z));
+    const float gradY = S.template ld<HEAT>(sidx(p, x, yp, z)) - S.template ld<HEAT>(sidx(p, x, ym, z));
+    const float gradZ = S.template ld<HEAT>(sidx(p, x, y, zp)) - S.template ld<HEAT>(sidx(p, x, y, zm));
+    const float uL = U[uidx(p, x, y, z)], uR = U[uidx(p, x + 1, y, z)];
+    const float vD = V[vidx(p, x, y, z)], vU = V[vidx(p, x, y + 1, z)];
+    const float wB = W[widx(p, x, y, z)], wF = W[widx(p, x, y, z + 1)];
+    const float curlX = (wF - wB) - (vU - vD);
+    const float curlY = (uR - uL) - (wF - wB);
+    const float curlZ = (vU - vD) - (uR - uL);
+    const float curlMag = fabsf(curlX) + fabsf(curlY) + fabsf(curlZ);
+    const float t = p.time;
+    const float noiseA = fbm2(u * 9.0f + w * 3.7f + t * 0.22f, v * 13.0f - t * 0.31f, p.saltNoise2) - 0.5f;
+    const float noiseB = fbm2(w * 10.0f - t * 0.19f, u * 8.0f + v * 5.5f + t * 0.27f, p.saltNoise2) - 0.5f;
+    const float thermal = sstep(0.04f, 4.20f, h + s * 0.28f);
+    const float upperDrag = sstep(0.26f, 0.94f, v);
+    const float lesGain = sqrtf(fmaxf(0.0f, k)) * (0.30f + c * 0.54f);
+    const float curlScale = sid == 2 ? 0.38f : (sid == 1 ? 1.18f : 1.0f);
+    const float buoyScale = sid == 2 ? 0.42f : (sid == 1 ? 1.06f : 1.0f);
+    const float curlGain = p.turbulence * curlScale * (0.18f + thermal * 0.82f + lesGain * 1.12f);
+    const float vortexGain = p.turbulence * curlScale * thermal * sat(curlMag * 0.46f) * 0.68f;
+    const float buoyFade = sid == 2 ? sstep(0.24f, 0.035f, v) : sstep(0.96f, 0.08f, v);
+    const float vertDamp = 0.5f * (vD + vU) * (0.10f + upperDrag * 0.72f + s * 0.020f);
+    const float buoyancy = h * buoyScale * (1.62f + p.turbulence * 0.34f + c * 0.16f) * buoyFade - s * 0.095f - gradY * 0.18f - vertDamp;
+    const float windProfile = 0.34f + v * 0.96f;
+    cv[idx] = p.dt * (buoyancy + curlY * vortexGain * 0.22f);
+    cu[idx] = p.dt * (p.windX * windProfile + (-gradZ * 1.36f + noiseA * (0.20f + upperDrag * 0.16f) + curlX * vortexGain) * curlGain);
+    cw[idx] = p.dt * (p.windZ * windProfile + (gradX * 1.36f + noiseB * (0.20f + upperDrag * 0.16f) + curlZ * vortexGain) * curlGain);
+}
+
+// New face values after the gathered force and the boundary clamp
+// (velocityBoundaryKernel :1261-1291). Both cells sharing a face evaluate the
+// same expression, so the face is bit-identical from either side.
+__device__ __forceinline__ float face_u(const float* U, const float* cu, const DevParams& p, int fx, int y, int z) {
+    if (fx == 0 || fx == p.nx) return 0.0f;
+    float val = U[uidx(p, fx, y, z)];
+    val += cu[sidx(p, fx - 1, y, z)];
+    val += cu[sidx(p, fx, y, z)];
+    return fminf(4.0f, fmaxf(-4.0f, val));
+}
+__device__ __forceinline__ float face_v(const float* V, const float* cv, const DevParams& p, int x, int fy, int z) {
+    if (fy == 0) return 0.0f;
+    float val = V[vidx(p, x, fy, z)];
+    val += cv[sidx(p, x, fy - 1, z)];
+    return fminf(6.4f, fmaxf(-0.9f, val));
+}
+__device__ __forceinline__ float face_w(const float* W, const float* cw, const DevParams& p, int x, int y, int fz) {
+    if (fz == 0 || fz == p.nz) return 0.0f;
+    float val = W[widx(p, x, y, fz)];
+    val += cw[sidx(p, x, y, fz - 1)];
+    val += cw[sidx(p, x, y, fz)];
+    return fminf(4.0f, fmaxf(-4.0f, val));
+}
+
+__global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                                                     const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                                     const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
+                                                     float* __restrict__ D, float* __restrict__ P, DevParams p) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= p.nx || y >= p.ny || z >= p.nz) return;
+    const float uL = face_u(U, cu, p, x, y, z), uR = face_u(U, cu, p, x + 1, y, z);
+    const float vD = face_v(V, cv, p, x, y, z), vU = face_v(V, cv, p, x, y + 1, z);
+    const float wB = face_w(W, cw, p, x, y, z), wF = face_w(W, cw, p, x, y, z + 1);
+    uo[uidx(p, x, y, z)] = uL;
+    if (x == p.nx - 1) uo[uidx(p, x + 1, y, z)] = uR;
+    vo[vidx(p, x, y, z)] = vD;
+    if (y == p.ny - 1) vo[vidx(p, x, y + 1, z)] = vU;
+    wo[widx(p, x, y, z)] = wB;
+    if (z == p.nz - 1) wo[widx(p, x, y, z + 1)] = wF;
+    const float div = uR - uL + vU - vD + wF - wB;
+    const size_t idx = sidx(p, x, y, z);
+    D[idx] = -div;
+    P[idx] = 0.0f;
+}
+
+// -------------------------------------------------------------- K12 ----
+// One red/black half-sweep, in place: the updated colour (x+y+z)&1 == parity
+// reads only the other colour, so every cell of a half-sweep is independent.
+__global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const float* __restrict__ D, int parity, float omega, int nx, int ny, int nz) {
+    const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const int x = 2 * tx + ((parity ^ y ^ z) & 1);
+    if (x >= nx || y >= ny || z >= nz) return;
+    const size_t row = static_cast<size_t>(nx);
+    const size_t plane = row * ny;
+    const size_t idx = static_cast<size_t>(z) * plane + static_cast<size_t>(y) * row + x;
+    const float pc = P[idx];
+    const float pL = x > 0 ? P[idx - 1] : pc;
+    const float pR = x < nx - 1 ? P[idx + 1] : pc;
+    const float pD = y > 0 ? P[idx - row] : pc;
+    const float pU = y < ny - 1 ? P[idx + row] : pc;
+    const float pB = z > 0 ? P[idx - plane] : pc;
+    const float pF = z < nz - 1 ? P[idx + plane] : pc;
+    const float projected = (D[idx] + pL + pR + pD + pU + pB + pF) * (1.0f / 6.0f);
+    P[idx] = pc + omega * (projected - pc);
+}
+
+// ----------------------------------------------------------- K13-K14 ----
+__global__ __launch_bounds__(256) void k_project(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, const float* __restrict__ P,
+                                                 DevParams p) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x > p.nx || y > p.ny || z > p.nz) return;
+    if (y < p.ny && z < p.nz) {
+        const size_t f = uidx(p, x, y, z);
+        if (x == 0 || x == p.nx) U[f] = 0.0f;
+        else U[f] -= P[sidx(p, x, y, z)] - P[sidx(p, x - 1, y, z)];
+    }
+    if (x < p.nx && z < p.nz) {
+        const size_t f = vidx(p, x, y, z);
+        if (y == 0) V[f] = 0.0f;
+        else if (y < p.ny) V[f] -= P[sidx(p, x, y, z)] - P[sidx(p, x, y - 1, z)];
+    }
+    if (x < p.nx && y < p.ny) {
+        const size_t f = widx(p, x, y, z);
+        if (z == 0 || z == p.nz) W[f] = 0.0f;
+        else W[f] -= P[sidx(p, x, y, z)] - P[sidx(p, x, y, z - 1)];
+    }
+}
+
+// -------------------------------------------------------------- reset ----
+template <bool B>
+__global__ __launch_bounds__(256) void k_reset(Scalars<B> S, float* __restrict__ P, float* __restrict__ D, DevParams p) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    if (x >= p.nx || y >= p.ny || z >= p.nz) return;
+    const size_t idx = sidx(p, x, y, z);
+    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
+    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
+    const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const float wx = (u - 0.50f) * 2.10f, wz = (w - 0.50f) * 1.64f;
+    const float material = bed_material(wx, wz, p);
+    float heat = 0, fuel = 0, soot = 0, ch = 0, ash = 0, pyro = 0, prog = 0, k = 0, so = 0;
+    if (material != 0.0f) {
+        const int sid = p.sceneId;
+        const float chunk = fbm2(wx * 18.0f + wz * 2.0f, wz * 22.0f - wx * 5.0f, p.saltNoise2);
+        const float bed = bed_source(material, wx, wz, v, p);
+        const float charScale = sid == 2 ? 0.004f : (sid == 1 ? 1.18f : 1.0f);
+        const float sootScale = sid == 2 ? 0.004f : (sid == 1 ? 0.92f : 1.0f);
+        heat = bed * (sid == 2 ? 1.34f : (sid == 1 ? 0.78f + chunk * 0.26f : 0.86f + chunk * 0.24f));
+        fuel = bed * (sid == 2 ? 1.42f : (sid == 1 ? 1.05f + chunk * 0.30f : 1.08f + chunk * 0.26f));
+        soot = bed * (sid == 2 ? 0.00065f : 0.018f) * sootScale;
+        ch = material * (0.92f + chunk * 0.72f) * charScale;
+        ash = material * (0.018f + sstep(0.56f, 0.86f, chunk) * 0.055f) * (sid == 2 ? 0.10f : 1.0f);
+        pyro = bed * (sid == 2 ? 0.018f : (sid == 1 ? 0.060f + chunk * 0.034f : 0.058f + chunk * 0.028f));
+        prog = bed * (sid == 2 ? 0.11f : (sid == 1 ? 0.24f : 0.21f));
+        k = bed * (sid == 2 ? 0.012f : (sid == 1 ? 0.082f : 0.065f));
+        so = bed * (sid == 2 ? 0.00050f : 0.018f) * sootScale;
+    }
+    S.template st<HEAT>(idx, heat);
+    S.template st<FUEL>(idx, fuel);
+    S.template st<OXYGEN>(idx, 1.0f);
+    S.template st<SOOT>(idx, soot);
+    S.template st<CHARF>(idx, ch);
+    S.template st<ASH>(idx, ash);
+    S.template st<PYRO>(idx, pyro);
+    S.template st<PROGRESS>(idx, prog);
+    S.template st<TURB>(idx, k);
+    S.template st<SOPT>(idx, so);
+    P[idx] = 0.0f;
+    D[idx] = 0.0f;
+}
+
+// ------------------------------------------------------------ metrics ----
+template <bool B>
+__global__ __launch_bounds__(256) void k_metrics(double* __restrict__ partial, Scalars<B> S, const float* __restrict__ U, const float* __restrict__ V,
+                                                 const float* __restrict__ W, const float* __restrict__ P, const float* __restrict__ D,
+                                                 const float4* __restrict__ L, const float* __restrict__ SH, DevParams p) {
+    double acc[kMetricSlots];
+#pragma unroll
+    for (int i = 0; i < kMetricSlots; ++i) acc[i] = 0.0;
+    const size_t cells = static_cast<size_t>(p.nx) * p.ny * p.nz;
+    for (size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < cells; idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const int x = static_cast<int>(idx % p.nx);
+        const int y = static_cast<int>((idx / p.nx) % p.ny);
+        const int z = static_cast<int>(idx / (static_cast<size_t>(p.nx) * p.ny));
+        const float heat = S.template ld<HEAT>(idx), fuel = S.template ld<FUEL>(idx), oxygen = S.template ld<OXYGEN>(idx);
+        const float soot = S.template ld<SOOT>(idx), ch = S.template ld<CHARF>(idx), ash = S.template ld<ASH>(idx);
+        const float pyro = S.template ld<PYRO>(idx), prog = S.template ld<PROGRESS>(idx), turb = S.template ld<TURB>(idx);
+        const float so = S.template ld<SOPT>(idx);
+        const float divBefore = fabsf(D[idx]);
+        const float div = U[uidx(p, x + 1, y, z)] - U[uidx(p, x, y, z)] + V[vidx(p, x, y + 1, z)] - V[vidx(p, x, y, z)] + W[widx(p, x, y, z + 1)] -
+                          W[widx(p, x, y, z)];
+        const float divAfter = fabsf(-div);
+        const float pc = P[idx];
+        const double resid = static_cast<double>(D[idx]) + P[sidx(p, max(x - 1, 0), y, z)] + P[sidx(p, min(x + 1, p.nx - 1), y, z)] +
+                             P[sidx(p, x, max(y - 1, 0), z)] + P[sidx(p, x, min(y + 1, p.ny - 1), z)] + P[sidx(p, x, y, max(z - 1, 0))] +
+                             P[sidx(p, x, y, min(z + 1, p.nz - 1))] - 6.0 * pc;
+        const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
+        const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
+        const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+        const float4 Lc = sample_light(L, p, u, v, w);
+        const float shadow = sample_shadow(SH, p, u, v, w);
+        if (!isfinite(heat) || !isfinite(fuel) || !isfinite(oxygen) || !isfinite(soot) || !isfinite(ch) || !isfinite(ash) || !isfinite(pyro) ||
+            !isfinite(prog) || !isfinite(turb) || !isfinite(so) || !isfinite(divBefore) || !isfinite(divAfter)) {
+            acc[31] += 1.0;
+            continue;
+        }
+        const float tempK = 293.0f + heat * 335.0f;
+        const float hrr = (fuel + pyro * 0.42f) * oxygen * sstep(610.0f, 1280.0f, tempK) * expf(-5200.0f / fmaxf(650.0f, tempK));
+        const float od = so + soot * (0.24f + 0.05f * sqrtf(fmaxf(0.0f, heat)));
+        const float nh = v;
+        const float flameMass = sstep(0.72f, 2.10f, heat + prog * 0.62f + pyro * 0.24f) * sstep(0.035f, 0.66f, oxygen) * sstep(0.012f, 0.52f, fuel + pyro * 0.36f);
+        const float smokeMass = sstep(0.035f, 1.16f, so + soot * 0.38f + ash * 0.12f) * sstep(0.08f, 0.88f, nh);
+        const float volShadow = shadow * expf(-Lc.w * 0.80f);
+        const float lum = lum3(f3(Lc.x, Lc.y, Lc.z));
+        const float marker = heat + fuel * 0.24f + prog * 0.55f;
+        const float sums[21] = {heat, fuel, oxygen, soot, ch, ash, pyro, prog, turb, so, lum, shadow, flameMass, smokeMass,
+                                flameMass * smokeMass, volShadow, lum * volShadow, od, hrr, 0.0f, 0.0f};
+#pragma unroll
+        for (int i = 0; i < 19; ++i) acc[i] += static_cast<double>(sums[i]);
+        acc[19] += static_cast<double>(divBefore) * divBefore;
+        acc[20] += static_cast<double>(divAfter) * divAfter;
+        acc[21] += resid * resid;
+        const float mx[8] = {heat, fuel, soot, pyro, prog, turb, divBefore, divAfter};
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (mx[i] > 0.0f) acc[22 + i] = fmax(acc[22 + i], static_cast<double>(mx[i]));
+        if (marker > 0.74f) acc[30] = fmax(acc[30], static_cast<double>(nh * 2.03f));
+    }
+    // block reduction: sums for 0..21 and 31, maxima for 22..30
+    __shared__ double lds[4][kMetricSlots];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < kMetricSlots; ++i) {
+        const bool isMax = i >= 22 && i <= 30;
+        const double r = isMax ? wave_max(acc[i]) : wave_sum(acc[i]);
+        if (lane == 0) lds[wave][i] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < kMetricSlots) {
+        const int i = threadIdx.x;
+        const bool isMax = i >= 22 && i <= 30;
+        const double r = isMax ? fmax(fmax(lds[0][i], lds[1][i]), fmax(lds[2][i], lds[3][i])) : (lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]);
+        partial[static_cast<size_t>(blockIdx.x) * kMetricSlots + i] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_metrics(const double* __restrict__ partial, int blocks, double* __restrict__ out) {
+    // 8 threads per slot, fixed order
+    const int slot = threadIdx.x >> 3, sub = threadIdx.x & 7;
+    const bool isMax = slot >= 22 && slot <= 30;
+    double r = 0.0;
+    for (int b = sub; b < blocks; b += 8) {
+        const double val = partial[static_cast<size_t>(b) * kMetricSlots + slot];
+        r = isMax ? fmax(r, val) : r + val;
+    }
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) {
+        const double o = __shfl_down(r, off, 8);
+        r = isMax ? fmax(r, o) : r + o;
+    }
+    if (sub == 0) out[slot] = r;
+}
+
+// --------------------------------------------------------- pack/unpack ----
+__global__ void k_pack_bf16(uint16_t* __restrict__ dst, const float* __restrict__ src, size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        uint32_t b = __float_as_uint(src[i]);
+        b += 0x7FFFu + ((b >> 16) & 1u);
+        dst[i] = static_cast<uint16_t>(b >> 16);
+    }
+}
+__global__ void k_unpack_bf16(float* __restrict__ dst, const uint16_t* __restrict__ src, size_t n) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = __uint_as_float(static_cast<uint32_t>(src[i]) << 16);
+}
+
+// ========================================================== launchers ====
+template <bool B>
+static Scalars<B> scal(void* const* f) {
+    Scalars<B> s;
+    for (int i = 0; i < NSCALAR; ++i) s.f[i] = f[i];
+    return s;
+}
+
+hipError_t launch_reset(const DeviceScene& s, const DevParams& p, hipStream_t st) {
+    const dim3 g = cell_grid(s.nx, s.ny, s.nz), b = cell_block();
+    if (s.bf16) k_reset<true><<<g, b, 0, st>>>(scal<true>(s.s), s.p, s.d, p);
+    else k_reset<false><<<g, b, 0, st>>>(scal<false>(s.s), s.p, s.d, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    const dim3 g = cell_grid(s.nx + 1, s.ny + 1, s.nz + 1), b = cell_block();
+    k_advect_velocity<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, p);
+    std::swap(s.u, s.uN);
+    std::swap(s.v, s.vN);
+    std::swap(s.w, s.wN);
+    return hipGetLastError();
+}
+
+hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    const dim3 g = cell_grid(s.nx, s.ny, s.nz), b = cell_block();
+    if (s.bf16) k_advect_react<true><<<g, b, 0, st>>>(scal<true>(s.sNext), scal<true>(s.s), s.u, s.v, s.w, p, s.partials);
+    else k_advect_react<false><<<g, b, 0, st>>>(scal<false>(s.sNext), scal<false>(s.s), s.u, s.v, s.w, p, s.partials);
+    k_finalize_release<<<1, 256, 0, st>>>(s.partials, react_partial_count(s.nx, s.ny, s.nz), s.acc);
+    for (int i = 0; i < NSCALAR; ++i) std::swap(s.s[i], s.sNext[i]);
+    return hipGetLastError();
+}
+
+hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    const dim3 g = cell_grid(s.nx, s.ny, s.nz), b = cell_block();
+    if (s.bf16) k_force_contrib<true><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, scal<true>(s.s), s.u, s.v, s.w, p);
+    else k_force_contrib<false><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, scal<false>(s.s), s.u, s.v, s.w, p);
+    k_force_apply<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, s.cu, s.cv, s.cw, s.d, s.p, p);
+    std::swap(s.u, s.uN);
+    std::swap(s.v, s.vN);
+    std::swap(s.w, s.wN);
+    return hipGetLastError();
+}
+
+hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    const dim3 b = cell_block();
+    const dim3 g(((s.nx + 1) / 2 + b.x - 1) / b.x, (s.ny + b.y - 1) / b.y, (s.nz + b.z - 1) / b.z);
+    for (int i = 0; i < p.iterations; ++i) {
+        k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 0, p.omega, s.nx, s.ny, s.nz);
+        k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 1, p.omega, s.nx, s.ny, s.nz);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_project(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    const dim3 g = cell_grid(s.nx + 1, s.ny + 1, s.nz + 1), b = cell_block();
+    k_project<<<g, b, 0, st>>>(s.u, s.v, s.w, s.p, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_metrics(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    if (s.partialCapacity < static_cast<size_t>(kMetricBlocks) * kMetricSlots) return hipErrorInvalidValue;
+    if (s.bf16)
+        k_metrics<true><<<kMetricBlocks, 256, 0, st>>>(s.partials, scal<true>(s.s), s.u, s.v, s.w, s.p, s.d, s.light, s.shadow, p);
+    else
+        k_metrics<false><<<kMetricBlocks, 256, 0, st>>>(s.partials, scal<false>(s.s), s.u, s.v, s.w, s.p, s.d, s.light, s.shadow, p);
+    k_reduce_metrics<<<1, 256, 0, st>>>(s.partials, kMetricBlocks, s.metricsOut);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_scalar(const DeviceScene& s, int field, const float* src, hipStream_t st) {
+    const size_t n = static_cast<size_t>(s.nx) * s.ny * s.nz;
+    k_pack_bf16<<<1024, 256, 0, st>>>(static_cast<uint16_t*>(s.s[field]), src, n);
+    return hipGetLastError();
+}
+hipError_t launch_unpack_scalar(const DeviceScene& s, int field, float* dst, hipStream_t st) {
+    const size_t n = static_cast<size_t>(s.nx) * s.ny * s.nz;
+    k_unpack_bf16<<<1024, 256, 0, st>>>(dst, static_cast<const uint16_t*>(s.s[field]), n);
+    return hipGetLastError();
+}
+
+}  // namespace fs

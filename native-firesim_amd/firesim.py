@@ -1,0 +1,340 @@
+"""ctypes bindings of include/firesim.h (the C ABI of libfiresim.so).
+
+The library is the MI355X HIP implementation of the reference's per-frame
+step (src/fire_cuda.h:103-119 is the interface it replaces). Loading fails
+loudly when the built library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, Structure, byref, c_char_p, c_float, c_int32, c_uint32, c_uint64, c_void_p, c_double
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfiresim.so")
+
+FS_OK = 0
+
+# enum fs_field_id
+FIELD_NAMES = [
+    "heat", "fuel", "oxygen", "soot", "char", "ash", "pyrolysis", "progress",
+    "turbulence", "soot_optics", "u", "v", "w", "pressure", "divergence", "light", "shadow",
+]
+FIELD = {name: i for i, name in enumerate(FIELD_NAMES)}
+SCALAR_FIELDS = FIELD_NAMES[:10]
+
+# enum fs_stage
+STAGE_ADVECT_VELOCITY, STAGE_ADVECT_REACT, STAGE_FORCE, STAGE_PRESSURE, STAGE_PROJECT, STAGE_LIGHT = range(6)
+STAGE_NAMES = ["advect_velocity", "advect_react", "force", "pressure", "project", "light"]
+
+STORAGE_FP32, STORAGE_BF16 = 0, 1
+SCENE_ROOM_TRAY, SCENE_CAMPFIRE, SCENE_GAS_BURNER, SCENE_EMITTERS_ONLY = 0, 1, 2, 3
+MAX_EMITTERS = 16
+
+
+class SceneDesc(Structure):
+    _fields_ = [
+        ("nx", c_int32), ("ny", c_int32), ("nz", c_int32),
+        ("frame_w", c_int32), ("frame_h", c_int32),
+        ("storage", c_int32), ("device", c_int32),
+    ]
+
+
+class Params(Structure):
+    _fields_ = [
+        ("dt", c_float),
+        ("mouse_x", c_float), ("mouse_y", c_float),
+        ("left_down", c_int32), ("right_down", c_int32),
+        ("scene_id", c_int32),
+        ("wind_x", c_float), ("wind_z", c_float),
+        ("turbulence", c_float), ("detail", c_float), ("smoke", c_float), ("intensity", c_float),
+        ("camera_yaw", c_float), ("camera_pitch", c_float), ("camera_distance", c_float),
+        ("cinematic_mode", c_int32), ("raymarch_steps", c_int32), ("render_debug_mode", c_int32),
+        ("exposure", c_float), ("reflection_gain", c_float), ("smoke_darkness", c_float),
+        ("emitter_center_x", c_float), ("emitter_center_z", c_float),
+        ("emitter_height_norm", c_float), ("emitter_height_band_norm", c_float), ("emitter_radius", c_float),
+        ("burner_center_count", c_int32),
+        ("burner_center_x", c_float * 4), ("burner_center_y", c_float * 4), ("burner_center_z", c_float * 4),
+        ("sor_omega", c_float), ("sor_iterations", c_int32),
+        ("noise_seed", c_uint32),
+    ]
+
+
+class Emitter(Structure):
+    _fields_ = [
+        ("cx", c_float), ("cy", c_float), ("cz", c_float), ("radius", c_float),
+        ("fuel_rate", c_float), ("temperature_k", c_float),
+    ]
+
+
+_METRIC_DOUBLES = [
+    "time_seconds",
+    "heat_sum", "fuel_sum", "oxygen_sum", "soot_sum", "char_sum", "ash_sum",
+    "pyrolysis_sum", "progress_sum", "turbulence_energy_sum", "soot_optical_depth_sum",
+    "max_heat", "max_fuel", "max_soot", "max_pyrolysis", "max_progress", "max_turbulence_energy",
+    "flame_height_meters",
+    "mean_optical_depth", "mean_scene_light", "mean_scene_shadow",
+    "flame_mass_proxy", "smoke_mass_proxy", "flame_smoke_overlap_proxy",
+    "mean_volumetric_shadow", "mean_room_irradiance", "heat_release_proxy",
+    "divergence_before_l2", "divergence_before_max",
+    "divergence_after_l2", "divergence_after_max", "divergence_reduction",
+    "heat_release_step", "heat_release_total", "sor_residual_l2",
+]
+
+
+class Metrics(Structure):
+    _fields_ = [
+        ("frame_index", c_int32),
+        ("grid_x", c_int32), ("grid_y", c_int32), ("grid_z", c_int32),
+        ("pressure_iterations", c_int32), ("invalid_cells", c_int32),
+    ] + [(name, c_double) for name in _METRIC_DOUBLES]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def default_params(**overrides) -> Params:
+    """FireSettings defaults (src/fire_cuda.h:65-101) plus the SOR constants
+    (src/fire_cuda.cu:104-105); keyword overrides set fields by name."""
+    p = Params()
+    p.dt = 1.0 / 60.0
+    p.mouse_x, p.mouse_y = 0.5, 0.12
+    p.left_down = p.right_down = 0
+    p.scene_id = 0
+    p.wind_x = p.wind_z = 0.0
+    p.turbulence, p.detail, p.smoke, p.intensity = 0.72, 0.88, 0.92, 1.0
+    p.camera_yaw, p.camera_pitch, p.camera_distance = 0.0, 0.18, 3.35
+    p.cinematic_mode, p.raymarch_steps, p.render_debug_mode = 0, 56, 0
+    p.exposure, p.reflection_gain, p.smoke_darkness = 1.0, 1.0, 1.0
+    p.emitter_center_x = p.emitter_center_z = 0.0
+    p.emitter_height_norm, p.emitter_height_band_norm, p.emitter_radius = 0.0, 0.06, 0.48
+    p.burner_center_count = 0
+    p.sor_omega, p.sor_iterations = 1.52, 40
+    p.noise_seed = 0
+    for k, v in overrides.items():
+        if k in ("burner_center_x", "burner_center_y", "burner_center_z"):
+            arr = getattr(p, k)
+            for i, val in enumerate(v):
+                arr[i] = val
+        else:
+            if not hasattr(p, k):
+                raise AttributeError(f"fs_params has no field {k!r}")
+            setattr(p, k, v)
+    return p
+
+
+def params_to_dict(p: Params) -> dict:
+    out = {}
+    for name, _ in p._fields_:
+        val = getattr(p, name)
+        out[name] = list(val) if name.startswith("burner_center_") and name != "burner_center_count" else val
+    return out
+
+
+def light_dims(nx: int, ny: int, nz: int) -> tuple[int, int, int]:
+    """clamp((n+3)/4, lo, hi), src/fire_cuda.cu:2920-2922."""
+    c = lambda n, lo, hi: max(lo, min(hi, (n + 3) // 4))  # noqa: E731
+    return c(nx, 18, 48), c(ny, 20, 56), c(nz, 16, 36)
+
+
+def field_shape(field: int, nx: int, ny: int, nz: int) -> tuple:
+    """Numpy shape (z-major, x fastest) of a field."""
+    if field < 10 or field in (FIELD["pressure"], FIELD["divergence"]):
+        return (nz, ny, nx)
+    if field == FIELD["u"]:
+        return (nz, ny, nx + 1)
+    if field == FIELD["v"]:
+        return (nz, ny + 1, nx)
+    if field == FIELD["w"]:
+        return (nz + 1, ny, nx)
+    lnx, lny, lnz = light_dims(nx, ny, nz)
+    if field == FIELD["light"]:
+        return (lnz, lny, lnx, 4)
+    if field == FIELD["shadow"]:
+        return (lnz, lny, lnx)
+    raise ValueError(field)
+
+
+class FireSimError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libfiresim.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FireSimError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    P = POINTER
+    sigs = {
+        "fs_abi_version": (c_int32, []),
+        "fs_default_params": (None, [P(Params)]),
+        "fs_clamp_params": (c_int32, [P(Params), P(Params)]),
+        "fs_light_dims": (c_int32, [c_int32, c_int32, c_int32, P(c_int32), P(c_int32), P(c_int32)]),
+        "fs_scene_bytes": (c_int32, [P(SceneDesc), P(c_uint64)]),
+        "fs_write_bmp": (c_int32, [c_char_p, P(c_uint32), c_int32, c_int32]),
+        "fs_last_global_error": (c_char_p, []),
+        "fs_create": (c_int32, [P(SceneDesc), P(c_void_p)]),
+        "fs_destroy": (c_int32, [c_void_p]),
+        "fs_reset": (c_int32, [c_void_p]),
+        "fs_set_params": (c_int32, [c_void_p, P(Params)]),
+        "fs_get_params": (c_int32, [c_void_p, P(Params)]),
+        "fs_set_emitters": (c_int32, [c_void_p, P(Emitter), c_int32]),
+        "fs_last_error": (c_char_p, [c_void_p]),
+        "fs_step": (c_int32, [c_void_p, c_int32]),
+        "fs_run_stage": (c_int32, [c_void_p, c_int32]),
+        "fs_synchronize": (c_int32, [c_void_p]),
+        "fs_set_time": (c_int32, [c_void_p, c_float, c_float]),
+        "fs_get_time": (c_int32, [c_void_p, P(c_float), P(c_float), P(c_int32)]),
+        "fs_render_hdr": (c_int32, [c_void_p, P(c_float)]),
+        "fs_render_bgra8": (c_int32, [c_void_p, P(c_uint32)]),
+        "fs_tonemap": (c_int32, [c_void_p, P(c_float), P(c_uint32)]),
+        "fs_field_count": (c_int32, [c_void_p, c_int32, P(c_uint64)]),
+        "fs_read_field": (c_int32, [c_void_p, c_int32, P(c_float)]),
+        "fs_write_field": (c_int32, [c_void_p, c_int32, P(c_float)]),
+        "fs_metrics_collect": (c_int32, [c_void_p, P(Metrics)]),
+        "fs_stream": (c_void_p, [c_void_p]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list:
+    """Names declared in include/firesim.h, in declaration order."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "firesim.h")
+    text = open(hdr).read()
+    return re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(fs_[a-z0-9_]+)\s*\(", text, re.M)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Scene:
+    """One simulation handle (fs_scene) with numpy field access."""
+
+    def __init__(self, nx: int, ny: int, nz: int, frame_w: int = 64, frame_h: int = 64,
+                 storage: int = STORAGE_FP32, device: int = 0):
+        self.lib = load_library()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.frame_w, self.frame_h = frame_w, frame_h
+        desc = SceneDesc(nx, ny, nz, frame_w, frame_h, storage, device)
+        h = c_void_p()
+        st = self.lib.fs_create(byref(desc), byref(h))
+        if st != FS_OK:
+            raise FireSimError(f"fs_create failed ({st}): {self.lib.fs_last_global_error().decode()}")
+        self.h = h
+
+    def _check(self, st: int, what: str):
+        if st != FS_OK:
+            msg = self.lib.fs_last_error(self.h)
+            raise FireSimError(f"{what} failed ({st}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def reset(self):
+        self._check(self.lib.fs_reset(self.h), "fs_reset")
+
+    def set_params(self, p: Params):
+        self._check(self.lib.fs_set_params(self.h, byref(p)), "fs_set_params")
+
+    def get_params(self) -> Params:
+        p = Params()
+        self._check(self.lib.fs_get_params(self.h, byref(p)), "fs_get_params")
+        return p
+
+    def set_emitters(self, emitters):
+        arr = (Emitter * max(1, len(emitters)))(*[Emitter(*e) if not isinstance(e, Emitter) else e for e in emitters])
+        self._check(self.lib.fs_set_emitters(self.h, arr, len(emitters)), "fs_set_emitters")
+
+    def step(self, n: int = 1):
+        self._check(self.lib.fs_step(self.h, n), "fs_step")
+
+    def run_stage(self, stage: int):
+        self._check(self.lib.fs_run_stage(self.h, stage), "fs_run_stage")
+
+    def synchronize(self):
+        self._check(self.lib.fs_synchronize(self.h), "fs_synchronize")
+
+    def set_time(self, t: float, render_t: float):
+        self._check(self.lib.fs_set_time(self.h, t, render_t), "fs_set_time")
+
+    def get_time(self):
+        t, rt, fi = c_float(), c_float(), c_int32()
+        self._check(self.lib.fs_get_time(self.h, byref(t), byref(rt), byref(fi)), "fs_get_time")
+        return t.value, rt.value, fi.value
+
+    def shape(self, field: int):
+        return field_shape(field, self.nx, self.ny, self.nz)
+
+    def read(self, field) -> np.ndarray:
+        f = FIELD[field] if isinstance(field, str) else field
+        out = np.empty(self.shape(f), dtype=np.float32)
+        self._check(self.lib.fs_read_field(self.h, f, out.ctypes.data_as(POINTER(c_float))), "fs_read_field")
+        return out
+
+    def write(self, field, data):
+        f = FIELD[field] if isinstance(field, str) else field
+        arr = _f32(data).reshape(self.shape(f))
+        self._check(self.lib.fs_write_field(self.h, f, arr.ctypes.data_as(POINTER(c_float))), "fs_write_field")
+
+    def render_hdr(self) -> np.ndarray:
+        out = np.empty((self.frame_h, self.frame_w, 4), dtype=np.float32)
+        self._check(self.lib.fs_render_hdr(self.h, out.ctypes.data_as(POINTER(c_float))), "fs_render_hdr")
+        return out
+
+    def render_bgra8(self) -> np.ndarray:
+        out = np.empty((self.frame_h, self.frame_w), dtype=np.uint32)
+        self._check(self.lib.fs_render_bgra8(self.h, out.ctypes.data_as(POINTER(c_uint32))), "fs_render_bgra8")
+        return out
+
+    def tonemap(self, hdr: np.ndarray) -> np.ndarray:
+        src = _f32(hdr).reshape(self.frame_h, self.frame_w, 4)
+        out = np.empty((self.frame_h, self.frame_w), dtype=np.uint32)
+        self._check(self.lib.fs_tonemap(self.h, src.ctypes.data_as(POINTER(c_float)), out.ctypes.data_as(POINTER(c_uint32))),
+                    "fs_tonemap")
+        return out
+
+    def metrics(self) -> dict:
+        m = Metrics()
+        self._check(self.lib.fs_metrics_collect(self.h, byref(m)), "fs_metrics_collect")
+        return m.as_dict()
+
+    def stream(self) -> int:
+        return self.lib.fs_stream(self.h)
+
+
+def write_bmp(path: str, bgra: np.ndarray) -> None:
+    lib = load_library()
+    arr = np.ascontiguousarray(bgra, dtype=np.uint32)
+    h, w = arr.shape
+    st = lib.fs_write_bmp(path.encode(), arr.ctypes.data_as(POINTER(c_uint32)), w, h)
+    if st != FS_OK:
+        raise FireSimError(f"fs_write_bmp failed ({st}): {lib.fs_last_global_error().decode()}")
