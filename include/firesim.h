@@ -177,6 +177,37 @@ typedef struct fs_metrics {
     double sor_residual_l2;
 } fs_metrics;
 
+/* Per-pass GPU times of fs_step_timed, from HIP events recorded on the scene
+ * stream between passes (the reference's measured path records the same kind
+ * of per-pass cudaEvent intervals, src/fire_cuda.cu:3188-3491). */
+typedef struct fs_stage_times {
+    double stage_ms[FS_STAGE_COUNT]; /* summed over the timed steps */
+    double total_ms;                 /* first pass start to last pass end */
+    int32_t steps;
+    int32_t pressure_launches;       /* kernel launches in one pressure pass */
+} fs_stage_times;
+
+/* Pressure-solve kernels. All give bit-identical results (same per-cell FP32
+ * expression in the same red/black order); they differ in how many
+ * half-sweeps one launch performs from an LDS-resident tile of p (d read from
+ * global memory unless noted). */
+enum fs_sor_kernel {
+    FS_SOR_HALF_SWEEP = 0,   /* one launch per half-sweep, as the reference (:3280-3284) */
+    FS_SOR_TB_T20_K2 = 1,    /* 20^3 tiles, 2 half-sweeps per launch */
+    FS_SOR_TB_T16_K4 = 2,    /* 16^3 tiles, 4 half-sweeps per launch (default) */
+    FS_SOR_TB_T8_K4 = 3,     /* 8^3 tiles, 4 half-sweeps per launch */
+    FS_SOR_TB_T12_K6 = 4,    /* 12^3 tiles, 6 half-sweeps per launch */
+    FS_SOR_TB_T16_K4_DL = 5, /* 16^3 tiles, 4 half-sweeps, d also staged in LDS */
+    FS_SOR_TB_T18_K3 = 6,    /* 18^3 tiles, 3 half-sweeps per launch */
+    FS_SOR_KERNEL_COUNT = 7
+};
+
+/* Launch-shape choices of a scene (picked by the autotuner). */
+typedef struct fs_tuning {
+    int32_t sor_kernel; /* enum fs_sor_kernel */
+    int32_t use_graph;  /* 1: fs_step replays a captured HIP graph of the step (default); 0: direct launches */
+} fs_tuning;
+
 /* ---- host-only helpers (no device needed) ---- */
 int fs_abi_version(void);
 void fs_default_params(fs_params* out);
@@ -198,12 +229,21 @@ int fs_reset(fs_scene* scene);
 int fs_set_params(fs_scene* scene, const fs_params* params);
 int fs_get_params(const fs_scene* scene, fs_params* out);
 int fs_set_emitters(fs_scene* scene, const fs_emitter* emitters, int32_t count);
+int fs_set_tuning(fs_scene* scene, const fs_tuning* tuning);
+int fs_get_tuning(const fs_scene* scene, fs_tuning* out);
 const char* fs_last_error(const fs_scene* scene);
 
 /* ---- stepping (asynchronous on the scene stream) ---- */
 int fs_step(fs_scene* scene, int32_t nsteps);
 /* Runs one pass with the current time, without advancing time. */
 int fs_run_stage(fs_scene* scene, int32_t stage);
+/* fs_step with per-pass event timing (at most 4096 steps per call); blocks
+ * until the steps finish. step_ms, if not null, receives nsteps per-step GPU
+ * times. Replaces fireCudaStepAndRenderMeasured's timing half (:105). */
+int fs_step_timed(fs_scene* scene, int32_t nsteps, fs_stage_times* out, double* step_ms);
+/* fs_step that also returns each step's GPU time (HIP events between steps,
+ * production path, graph replay when enabled); blocks until done. */
+int fs_step_latency(fs_scene* scene, int32_t nsteps, double* step_ms);
 int fs_synchronize(fs_scene* scene);
 int fs_set_time(fs_scene* scene, float time, float render_time);
 int fs_get_time(const fs_scene* scene, float* time, float* render_time, int32_t* frame_index);

@@ -21,7 +21,7 @@ struct DevParams {
     int nx, ny, nz;
     int lnx, lny, lnz;
     int fw, fh;
-    float dt, time;
+    float dt;
     float mouseX, mouseY;
     int leftDown, rightDown, sceneId;
     float windX, windZ, turbulence, detail, smokeGain, intensity;
@@ -37,6 +37,10 @@ struct DevParams {
     float tanHalfFov, aspect;
     int emitterCount;
     float emitter[kMaxEmitters][6];  // cx, cy, cz, r (voxels), fuel rate, heat level
+    // Device clock [simulation time, render time]. Kernels read the time here
+    // rather than from this (by-value) block, so that a captured step graph
+    // stays valid while a one-thread kernel advances the clock every step.
+    const float* clock;
 };
 
 // Scalar field set. When BF16 is true, heat, fuel and soot are stored as
@@ -342,12 +346,12 @@ __device__ inline float bed_material(float x, float z, const DevParams& p) {
 }
 
 // src/fire_cuda.cu:795-803, given the material already evaluated.
-__device__ inline float bed_source(float material, float x, float z, float h, const DevParams& p) {
+__device__ inline float bed_source(float material, float x, float z, float h, const DevParams& p, float time) {
     if (material == 0.0f) return 0.0f;
     const float height = p.sceneId == 2
         ? sstep(fmaxf(0.005f, p.emitterHeightBandNorm * 0.34f), fmaxf(0.0015f, p.emitterHeightBandNorm * 0.055f), fabsf(h - p.emitterHeightNorm))
         : sstep(p.sceneId == 1 ? 0.245f : 0.145f, 0.00f, h);
-    const float breathing = 0.84f + 0.16f * fbm2(x * 9.0f + p.time * 0.035f, z * 11.0f - p.time * 0.025f, p.saltNoise2);
+    const float breathing = 0.84f + 0.16f * fbm2(x * 9.0f + time * 0.035f, z * 11.0f - time * 0.025f, p.saltNoise2);
     return sat(height * sat(material * breathing));
 }
 
@@ -376,6 +380,22 @@ __device__ inline F3 blackbody(float tempK) {
 __device__ __forceinline__ float domain_fade(float u, float v, float w) {
     const float side = sstep(0.000f, 0.095f, u) * sstep(1.000f, 0.905f, u) * sstep(0.000f, 0.100f, w) * sstep(1.000f, 0.900f, w);
     return sat(side * sstep(0.000f, 0.018f, v) * sstep(1.000f, 0.520f, v));
+}
+
+// ------------------------------------------------------ XCD mapping ----
+// MI355X dispatches the workgroups of a launch to its 8 XCDs round-robin by
+// linear workgroup id, and each XCD has its own L2. xcd_block() renumbers the
+// workgroups so that every XCD receives one contiguous range of logical blocks
+// (z-major, i.e. a slab of the domain): neighbouring tiles then share an L2 and
+// each XCD's working set is 1/8 of the field instead of all of it.
+constexpr unsigned kXcdCount = 8;
+__device__ __forceinline__ uint3 xcd_block() {
+    const unsigned gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+    const unsigned n = gx * gy * gz;
+    const unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned xcd = hw % kXcdCount, slot = hw / kXcdCount;
+    const unsigned logical = xcd * (n / kXcdCount) + min(xcd, n % kXcdCount) + slot;
+    return make_uint3(logical % gx, (logical / gx) % gy, logical / (gx * gy));
 }
 
 // ------------------------------------------------------------ reduce ----

@@ -22,6 +22,7 @@ struct DeviceScene {
     float *u = nullptr, *v = nullptr, *w = nullptr;
     float *uN = nullptr, *vN = nullptr, *wN = nullptr;
     float *p = nullptr, *d = nullptr;
+    float* p2 = nullptr;  // second pressure buffer: tile SOR launches read one, write the other
     float *cu = nullptr, *cv = nullptr, *cw = nullptr;  // per-cell force contributions
     float4 *light = nullptr, *lightNext = nullptr;
     float* shadow = nullptr;
@@ -31,6 +32,8 @@ struct DeviceScene {
     size_t partialCapacity = 0;  // doubles
     double* acc = nullptr;       // [0] heat release of the last step, [1] since reset
     double* metricsOut = nullptr;  // kMetricSlots doubles
+    float* clock = nullptr;        // [simulation time, render time]
+    int sorKernel = 2;           // enum fs_sor_kernel
 };
 
 // Grid and block shape of the cell-parallel passes (64 lanes along x).
@@ -38,6 +41,7 @@ dim3 cell_block();
 dim3 cell_grid(int nx, int ny, int nz);
 
 hipError_t launch_reset(const DeviceScene& s, const DevParams& p, hipStream_t st);
+hipError_t launch_advance_clock(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st);
@@ -52,5 +56,9 @@ hipError_t launch_pack_scalar(const DeviceScene& s, int field, const float* src,
 hipError_t launch_unpack_scalar(const DeviceScene& s, int field, float* dst, hipStream_t st);
 
 size_t react_partial_count(int nx, int ny, int nz);
+// Kernel launches one pressure pass issues for `iterations` SOR iterations.
+int pressure_launch_count(const DeviceScene& s, int iterations);
+// Half-sweeps one temporally blocked SOR launch performs (1 for the plain kernel).
+int sor_depth(int sorKernel);
 
 }  // namespace fs

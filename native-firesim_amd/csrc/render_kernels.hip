@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_light_build(float4* __restrict__ L, flo
     const float gy = sample_scalar<HEAT, B>(S, p, u, v + cellY, w) - sample_scalar<HEAT, B>(S, p, u, v - cellY, w);
     const float gz = sample_scalar<HEAT, B>(S, p, u, v, w + cellZ) - sample_scalar<HEAT, B>(S, p, u, v, w - cellZ);
     const float front = sqrtf(gx * gx + gy * gy + gz * gz) + fuel * oxygen * 0.035f + progress * 0.16f;
-    const float t = p.time;
+    const float t = p.clock[0];
     const float fine = fbm3(u * 34.0f + t * 0.45f, v * 42.0f - t * 0.82f, w * 34.0f + t * 0.22f, p.saltNoise3);
     const float shear = fbm3(u * 12.0f - t * 0.38f, v * 11.0f - t * 0.62f, w * 13.0f + t * 0.20f, p.saltNoise3);
     const float reactionFront = sstep(0.035f, 0.54f, front + pyro * 0.10f);
@@ -188,7 +188,8 @@ __device__ void source_model_overlay(float uvx, float uvy, const Cam& cam, const
 
 // roomBackgroundRay src/fire_cuda.cu:1683-1865 (floorGrid :1567-1573 is 0,
 // so the terms it scales are omitted).
-__device__ F3 room_background(const Cam& cam, float uvx, float uvy, F3 rd, float glow, const float4* L, const float* SH, const DevParams& p) {
+__device__ F3 room_background(const Cam& cam, float uvx, float uvy, F3 rd, float glow, const float4* L, const float* SH, const DevParams& p,
+                              float time) {
     const float roomHalf = 2.55f, ceilingY = 2.18f;
     const bool cinematic = p.cinematic != 0;
     float bestT = 1.0e6f;
@@ -317,7 +318,7 @@ __device__ F3 room_background(const Cam& cam, float uvx, float uvy, F3 rd, float
     const float corner = (surface == 1 || surface == 2) ? fminf(roomHalf - fabsf(hit.x), roomHalf - fabsf(hit.z))
                                                         : fminf(wallEdge, fminf(hit.y, ceilingY - hit.y));
     color = mul3(color, 0.84f + sstep(0.0f, 0.54f, corner) * 0.16f);
-    const float gx = floorf(hit.x * 19.0f + hit.z * 13.0f), gy = floorf(hit.y * 15.0f + p.time * 0.08f);
+    const float gx = floorf(hit.x * 19.0f + hit.z * 13.0f), gy = floorf(hit.y * 15.0f + time * 0.08f);
     const float grain = (hash2(static_cast<int>(gx), static_cast<int>(gy), p.saltGrain) - 0.5f) * 0.006f;
     color = add3(color, f3(grain, grain, grain));
     source_model_overlay(uvx, uvy, cam, p, color);
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(256) void k_render(float4* __restrict__ out, Scalar
     float trans = 1.0f, flameGlow = 0.0f, smokeOcc = 0.0f;
     float dbgFlame = 0.0f, dbgOD = 0.0f, dbgTemp = 0.0f, dbgFuel = 0.0f, dbgVel = 0.0f;
     const int sid = p.sceneId;
-    const float t = p.time;
+    const float t = p.clock[1];
     if (intersect_box(cam.eye, rd, boxNear, boxFar)) {
         const int steps = min(max(p.raymarchSteps, 24), 256);
         const float t0 = fmaxf(0.0f, boxNear);
@@ -561,7 +562,7 @@ __global__ __launch_bounds__(256) void k_render(float4* __restrict__ out, Scalar
         return;
     }
     F3 color = accum;
-    if (trans > 0.006f) color = add3(mul3(room_background(cam, uvx, uvy, rd, baseGlow, L, SH, p), trans), accum);
+    if (trans > 0.006f) color = add3(mul3(room_background(cam, uvx, uvy, rd, baseGlow, L, SH, p, t), trans), accum);
     const float bloom = sat(flameGlow * 0.10f) * (1.0f - sat(smokeOcc * 0.95f));
     color = add3(color, mul3(f3(1.0f, 0.25f, 0.045f), bloom * bloom * 0.017f * p.reflectionGain));
     out[o] = make_float4(fmaxf(0.0f, color.x), fmaxf(0.0f, color.y), fmaxf(0.0f, color.z), 1.0f);

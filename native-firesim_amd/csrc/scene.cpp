@@ -116,7 +116,7 @@ std::vector<size_t> buffer_sizes(const Dims& d) {
         b.push_back(vc * 4);
         b.push_back(wc * 4);
     }
-    for (int k = 0; k < 5; ++k) b.push_back(c * 4);  // p, d, cu, cv, cw
+    for (int k = 0; k < 6; ++k) b.push_back(c * 4);  // p, p2, d, cu, cv, cw
     b.push_back(d.lcells() * 16);
     b.push_back(d.lcells() * 16);
     b.push_back(d.lcells() * 4);
@@ -125,6 +125,7 @@ std::vector<size_t> buffer_sizes(const Dims& d) {
     b.push_back(d.partials() * 8);
     b.push_back(2 * 8);
     b.push_back(fs::kMetricSlots * 8);
+    b.push_back(2 * 4);  // device clock
     return b;
 }
 
@@ -184,6 +185,15 @@ struct fs_scene {
     float time = 0.0f, renderTime = 0.0f;
     int frameIndex = 0;
     std::string lastError = "no error";
+    // Step graphs. Scalar fields and (for an odd number of SOR launches) the
+    // pressure buffers change places every step, so two one-step graphs are
+    // captured, from the buffer assignment before an even and an odd step;
+    // state[g] is the assignment graph g starts from.
+    bool useGraph = true;
+    bool graphsValid = false;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    DeviceScene state[2];
+    int parity = 0;  // graph to run next
 
     int fail(int status, const std::string& msg) {
         lastError = msg;
@@ -221,7 +231,7 @@ void camera_of(const fs_params& s, int fw, int fh, DevParams& p) {
     p.aspect = static_cast<float>(fw) / static_cast<float>(fh);
 }
 
-DevParams dev_params(const fs_scene& sc, float time) {
+DevParams dev_params(const fs_scene& sc) {
     DevParams p;
     std::memset(&p, 0, sizeof(p));
     const fs_params& s = sc.prm;
@@ -230,7 +240,7 @@ DevParams dev_params(const fs_scene& sc, float time) {
     p.lnx = d.lnx; p.lny = d.lny; p.lnz = d.lnz;
     p.fw = d.fw; p.fh = d.fh;
     p.dt = s.dt;
-    p.time = time;
+    p.clock = sc.ds.clock;
     p.mouseX = s.mouse_x; p.mouseY = s.mouse_y;
     p.leftDown = s.left_down; p.rightDown = s.right_down;
     p.sceneId = s.scene_id;
@@ -286,6 +296,88 @@ int run_stage(fs_scene* s, int stage, const DevParams& p) {
         default: return s->fail(FS_ERR_INVALID_ARGUMENT, "unknown stage");
     }
     return e == hipSuccess ? FS_OK : s->hip_fail(what, e);
+}
+
+// One step's device work: advance the device clock, then the six passes.
+int enqueue_step(fs_scene* s, const DevParams& p) {
+    hipError_t e = fs::launch_advance_clock(s->ds, p, s->stream);
+    if (e != hipSuccess) return s->hip_fail("advance clock", e);
+    for (int stage = 0; stage < FS_STAGE_COUNT; ++stage)
+        if (int st = run_stage(s, stage, p)) return st;
+    return FS_OK;
+}
+
+void drop_graphs(fs_scene* s) {
+    for (auto& g : s->exec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    s->graphsValid = false;
+}
+
+bool same_buffers(const DeviceScene& a, const DeviceScene& b) {
+    for (int i = 0; i < fs::NSCALAR; ++i)
+        if (a.s[i] != b.s[i]) return false;
+    return a.u == b.u && a.v == b.v && a.w == b.w && a.p == b.p && a.light == b.light;
+}
+
+// Captures the step from the current buffer assignment (graph 0) and from the
+// assignment one step later (graph 1); two steps restore the assignment.
+int capture_graphs(fs_scene* s) {
+    drop_graphs(s);
+    const DevParams p = dev_params(*s);
+    const DeviceScene base = s->ds;
+    for (int g = 0; g < 2; ++g) {
+        s->state[g] = s->ds;
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            s->ds = base;
+            return s->hip_fail("hipStreamBeginCapture", e);
+        }
+        const int st = enqueue_step(s, p);
+        e = hipStreamEndCapture(s->stream, &graph);
+        if (st != FS_OK || e != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            s->ds = base;
+            drop_graphs(s);
+            return st != FS_OK ? st : s->hip_fail("hipStreamEndCapture", e);
+        }
+        e = hipGraphInstantiate(&s->exec[g], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            s->ds = base;
+            drop_graphs(s);
+            return s->hip_fail("hipGraphInstantiate", e);
+        }
+    }
+    if (!same_buffers(s->ds, base)) {
+        s->ds = base;
+        drop_graphs(s);
+        return s->fail(FS_ERR_HIP, "step graph capture: two steps did not restore the buffer assignment");
+    }
+    s->ds = base;
+    s->parity = 0;
+    s->graphsValid = true;
+    return FS_OK;
+}
+
+// Enqueues one step (graph replay or direct launches) and advances the host clock.
+int one_step(fs_scene* s) {
+    s->renderTime += s->prm.dt;
+    s->time += s->prm.dt;
+    int st = FS_OK;
+    if (s->useGraph) {
+        if (!s->graphsValid && (st = capture_graphs(s)) != FS_OK) return st;
+        const hipError_t e = hipGraphLaunch(s->exec[s->parity], s->stream);
+        if (e != hipSuccess) return s->hip_fail("hipGraphLaunch", e);
+        s->ds = s->state[s->parity ^ 1];
+        s->parity ^= 1;
+    } else {
+        st = enqueue_step(s, dev_params(*s));
+    }
+    ++s->frameIndex;
+    return st;
 }
 
 // Element count of a field, and whether it is a scalar.
@@ -424,7 +516,7 @@ int fs_create(const fs_scene_desc* desc, fs_scene** out) {
     for (int i = 0; i < fs::NSCALAR; ++i) ds.sNext[i] = take();
     ds.u = static_cast<float*>(take()); ds.v = static_cast<float*>(take()); ds.w = static_cast<float*>(take());
     ds.uN = static_cast<float*>(take()); ds.vN = static_cast<float*>(take()); ds.wN = static_cast<float*>(take());
-    ds.p = static_cast<float*>(take()); ds.d = static_cast<float*>(take());
+    ds.p = static_cast<float*>(take()); ds.p2 = static_cast<float*>(take()); ds.d = static_cast<float*>(take());
     ds.cu = static_cast<float*>(take()); ds.cv = static_cast<float*>(take()); ds.cw = static_cast<float*>(take());
     ds.light = static_cast<float4*>(take()); ds.lightNext = static_cast<float4*>(take());
     ds.shadow = static_cast<float*>(take());
@@ -434,6 +526,7 @@ int fs_create(const fs_scene_desc* desc, fs_scene** out) {
     ds.partialCapacity = s->dims.partials();
     ds.acc = static_cast<double*>(take());
     ds.metricsOut = static_cast<double*>(take());
+    ds.clock = static_cast<float*>(take());
     fs_default_params(&s->prm);
     const int st = fs_reset(s);
     if (st != FS_OK) {
@@ -448,10 +541,9 @@ int fs_create(const fs_scene_desc* desc, fs_scene** out) {
 int fs_destroy(fs_scene* s) {
     if (!s) return FS_OK;
     (void)hipSetDevice(s->device);
-    if (s->stream) {
-        (void)hipStreamSynchronize(s->stream);
-        (void)hipStreamDestroy(s->stream);
-    }
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    drop_graphs(s);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->pool) (void)hipFree(s->pool);
     delete s;
     return FS_OK;
@@ -465,7 +557,7 @@ int fs_reset(fs_scene* s) {
     // Everything except the scalars is zero at reset; clear the whole pool.
     hipError_t e = hipMemsetAsync(s->pool, 0, s->poolBytes, s->stream);
     if (e != hipSuccess) return s->hip_fail("reset memset", e);
-    const DevParams p = dev_params(*s, 0.0f);
+    const DevParams p = dev_params(*s);
     e = fs::launch_reset(s->ds, p, s->stream);
     if (e != hipSuccess) return s->hip_fail("reset kernel", e);
     e = hipStreamSynchronize(s->stream);
@@ -480,6 +572,7 @@ int fs_set_params(fs_scene* s, const fs_params* prm) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
     if (!prm) return s->fail(FS_ERR_INVALID_ARGUMENT, "null params");
     clamp_params(*prm, s->prm);
+    drop_graphs(s);
     return FS_OK;
 }
 
@@ -500,6 +593,25 @@ int fs_set_emitters(fs_scene* s, const fs_emitter* e, int32_t count) {
             return s->fail(FS_ERR_INVALID_ARGUMENT, "emitter values must be finite with radius >= 0");
     }
     s->emitters.assign(e, e + count);
+    drop_graphs(s);
+    return FS_OK;
+}
+
+int fs_set_tuning(fs_scene* s, const fs_tuning* t) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (!t) return s->fail(FS_ERR_INVALID_ARGUMENT, "null tuning");
+    if (t->sor_kernel < 0 || t->sor_kernel >= FS_SOR_KERNEL_COUNT) return s->fail(FS_ERR_INVALID_ARGUMENT, "unknown SOR kernel");
+    s->ds.sorKernel = t->sor_kernel;
+    s->useGraph = t->use_graph != 0;
+    drop_graphs(s);
+    return FS_OK;
+}
+
+int fs_get_tuning(const fs_scene* s, fs_tuning* out) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (!out) return global_fail(FS_ERR_INVALID_ARGUMENT, "null output");
+    out->sor_kernel = s->ds.sorKernel;
+    out->use_graph = s->useGraph ? 1 : 0;
     return FS_OK;
 }
 
@@ -511,22 +623,110 @@ int fs_step(fs_scene* s, int32_t nsteps) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
     if (nsteps < 0) return s->fail(FS_ERR_INVALID_ARGUMENT, "negative step count");
     if (int st = set_device(s)) return st;
-    for (int i = 0; i < nsteps; ++i) {
+    for (int i = 0; i < nsteps; ++i)
+        if (int st = one_step(s)) return st;
+    return FS_OK;
+}
+
+int fs_step_latency(fs_scene* s, int32_t nsteps, double* step_ms) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (!step_ms) return s->fail(FS_ERR_INVALID_ARGUMENT, "null latency output");
+    if (nsteps < 0 || nsteps > 4096) return s->fail(FS_ERR_INVALID_ARGUMENT, "step count out of range [0, 4096]");
+    if (int st = set_device(s)) return st;
+    std::vector<hipEvent_t> ev(static_cast<size_t>(nsteps) + 1, nullptr);
+    int status = FS_OK;
+    for (auto& e : ev) {
+        const hipError_t err = hipEventCreate(&e);
+        if (err != hipSuccess) {
+            status = s->hip_fail("hipEventCreate", err);
+            break;
+        }
+    }
+    if (status == FS_OK && s->useGraph && !s->graphsValid) status = capture_graphs(s);
+    for (int i = 0; status == FS_OK && i < nsteps; ++i) {
+        hipError_t err = hipEventRecord(ev[i], s->stream);
+        if (err != hipSuccess) status = s->hip_fail("hipEventRecord", err);
+        if (status == FS_OK) status = one_step(s);
+    }
+    if (status == FS_OK && nsteps > 0) {
+        hipError_t err = hipEventRecord(ev[nsteps], s->stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(s->stream);
+        if (err != hipSuccess) status = s->hip_fail("latency sync", err);
+    }
+    for (int i = 0; status == FS_OK && i < nsteps; ++i) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+        step_ms[i] = ms;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return status;
+}
+
+int fs_step_timed(fs_scene* s, int32_t nsteps, fs_stage_times* out, double* step_ms) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (!out) return s->fail(FS_ERR_INVALID_ARGUMENT, "null timing output");
+    if (nsteps < 0 || nsteps > 4096) return s->fail(FS_ERR_INVALID_ARGUMENT, "step count out of range [0, 4096]");
+    if (int st = set_device(s)) return st;
+    std::memset(out, 0, sizeof(*out));
+    const int per = FS_STAGE_COUNT + 1;
+    std::vector<hipEvent_t> ev(static_cast<size_t>(nsteps) * per, nullptr);
+    int status = FS_OK;
+    for (auto& e : ev) {
+        const hipError_t err = hipEventCreate(&e);
+        if (err != hipSuccess) {
+            status = s->hip_fail("hipEventCreate", err);
+            break;
+        }
+    }
+    drop_graphs(s);  // direct launches change the buffer assignment
+    for (int i = 0; status == FS_OK && i < nsteps; ++i) {
         s->renderTime += s->prm.dt;
         s->time += s->prm.dt;
-        const DevParams p = dev_params(*s, s->time);
-        for (int stage = 0; stage < FS_STAGE_COUNT; ++stage)
-            if (int st = run_stage(s, stage, p)) return st;
+        const DevParams p = dev_params(*s);
+        hipEvent_t* e = &ev[static_cast<size_t>(i) * per];
+        hipError_t err = fs::launch_advance_clock(s->ds, p, s->stream);
+        if (err == hipSuccess) err = hipEventRecord(e[0], s->stream);
+        for (int stage = 0; err == hipSuccess && status == FS_OK && stage < FS_STAGE_COUNT; ++stage) {
+            status = run_stage(s, stage, p);
+            if (status == FS_OK) err = hipEventRecord(e[stage + 1], s->stream);
+        }
+        if (err != hipSuccess) status = s->hip_fail("hipEventRecord", err);
         ++s->frameIndex;
     }
-    return FS_OK;
+    if (status == FS_OK) {
+        const hipError_t err = hipStreamSynchronize(s->stream);
+        if (err != hipSuccess) status = s->hip_fail("timed step sync", err);
+    }
+    for (int i = 0; status == FS_OK && i < nsteps; ++i) {
+        hipEvent_t* e = &ev[static_cast<size_t>(i) * per];
+        for (int stage = 0; stage < FS_STAGE_COUNT; ++stage) {
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e[stage], e[stage + 1]);
+            out->stage_ms[stage] += ms;
+        }
+        float stepMs = 0.0f;
+        (void)hipEventElapsedTime(&stepMs, e[0], e[FS_STAGE_COUNT]);
+        if (step_ms) step_ms[i] = stepMs;
+    }
+    if (status == FS_OK && nsteps > 0) {
+        float totalMs = 0.0f;
+        (void)hipEventElapsedTime(&totalMs, ev[0], ev[static_cast<size_t>(nsteps) * per - 1]);
+        out->total_ms = totalMs;
+    }
+    out->steps = nsteps;
+    out->pressure_launches = fs::pressure_launch_count(s->ds, s->prm.sor_iterations);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return status;
 }
 
 int fs_run_stage(fs_scene* s, int32_t stage) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
     if (stage < 0 || stage >= FS_STAGE_COUNT) return s->fail(FS_ERR_INVALID_ARGUMENT, "unknown stage");
     if (int st = set_device(s)) return st;
-    return run_stage(s, stage, dev_params(*s, s->time));
+    drop_graphs(s);  // direct launches change the buffer assignment
+    return run_stage(s, stage, dev_params(*s));
 }
 
 int fs_synchronize(fs_scene* s) {
@@ -538,6 +738,11 @@ int fs_synchronize(fs_scene* s) {
 
 int fs_set_time(fs_scene* s, float time, float renderTime) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (int st = set_device(s)) return st;
+    const float clock[2] = {time, renderTime};
+    hipError_t e = hipMemcpyAsync(s->ds.clock, clock, sizeof(clock), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return s->hip_fail("fs_set_time", e);
     s->time = time;
     s->renderTime = renderTime;
     return FS_OK;
@@ -555,7 +760,7 @@ int fs_render_hdr(fs_scene* s, float* rgba) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
     if (!rgba) return s->fail(FS_ERR_INVALID_ARGUMENT, "output pixel pointer was null");
     if (int st = set_device(s)) return st;
-    const DevParams p = dev_params(*s, s->renderTime);
+    const DevParams p = dev_params(*s);
     hipError_t e = fs::launch_render(s->ds, p, s->stream);
     if (e != hipSuccess) return s->hip_fail("render launch", e);
     e = hipMemcpyAsync(rgba, s->ds.hdr, s->dims.pixels() * 16, hipMemcpyDeviceToHost, s->stream);
@@ -567,7 +772,7 @@ int fs_render_bgra8(fs_scene* s, uint32_t* bgra) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
     if (!bgra) return s->fail(FS_ERR_INVALID_ARGUMENT, "output pixel pointer was null");
     if (int st = set_device(s)) return st;
-    const DevParams p = dev_params(*s, s->renderTime);
+    const DevParams p = dev_params(*s);
     hipError_t e = fs::launch_render(s->ds, p, s->stream);
     if (e == hipSuccess) e = fs::launch_tonemap(s->ds, p, s->stream);
     if (e != hipSuccess) return s->hip_fail("render launch", e);
@@ -580,7 +785,7 @@ int fs_tonemap(fs_scene* s, const float* rgba, uint32_t* bgra) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
     if (!rgba || !bgra) return s->fail(FS_ERR_INVALID_ARGUMENT, "null pixel buffer");
     if (int st = set_device(s)) return st;
-    const DevParams p = dev_params(*s, s->renderTime);
+    const DevParams p = dev_params(*s);
     hipError_t e = hipMemcpyAsync(s->ds.hdr, rgba, s->dims.pixels() * 16, hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) e = fs::launch_tonemap(s->ds, p, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(bgra, s->ds.bgra, s->dims.pixels() * 4, hipMemcpyDeviceToHost, s->stream);
@@ -633,7 +838,7 @@ int fs_metrics_collect(fs_scene* s, fs_metrics* m) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
     if (!m) return s->fail(FS_ERR_INVALID_ARGUMENT, "metrics output pointer was null");
     if (int st = set_device(s)) return st;
-    const DevParams p = dev_params(*s, s->time);
+    const DevParams p = dev_params(*s);
     hipError_t e = fs::launch_metrics(s->ds, p, s->stream);
     double slots[fs::kMetricSlots];
     double acc[2];

@@ -52,9 +52,10 @@ __device__ __forceinline__ double block_sum_256(double v) {
 __global__ __launch_bounds__(256) void k_advect_velocity(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
                                                          const float* __restrict__ U, const float* __restrict__ V,
                                                          const float* __restrict__ W, DevParams p) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const uint3 blk = xcd_block();
+    const int x = blk.x * blockDim.x + threadIdx.x;
+    const int y = blk.y * blockDim.y + threadIdx.y;
+    const int z = blk.z * blockDim.z + threadIdx.z;
     if (x > p.nx || y > p.ny || z > p.nz) return;
     const float fnx = static_cast<float>(p.nx), fny = static_cast<float>(p.ny), fnz = static_cast<float>(p.nz);
     if (y < p.ny && z < p.nz) {
@@ -114,6 +115,7 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
     const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
     const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
     const float dt = p.dt;
+    const float time = p.clock[0];
     const int sid = p.sceneId;
     const float h0 = in.template ld<HEAT>(idx), f0 = in.template ld<FUEL>(idx), o0 = in.template ld<OXYGEN>(idx);
     const float s0 = in.template ld<SOOT>(idx), ch0 = in.template ld<CHARF>(idx), a0 = in.template ld<ASH>(idx);
@@ -164,11 +166,11 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
     const float exposedChar = sat(charMass / fmaxf(0.055f, charMass + ash * 1.75f));
     const float charEdges = sstep(0.06f, 0.78f, material) * sstep(0.92f, 0.02f, ash);
     const float gasFeed = sid == 2 ? 1.18f : exposedChar * charEdges;
-    const float bedSrc = bed_source(material, worldX, worldZ, v, p);
+    const float bedSrc = bed_source(material, worldX, worldZ, v, p, time);
     // The source noise only matters when a bed is exposed or char can release gas.
     float sourceNoise = 0.70f;
     if (bedSrc * gasFeed != 0.0f || charMass != 0.0f) {
-        sourceNoise = 0.70f + 0.48f * fbm2(worldX * 13.0f + worldZ * 3.0f + p.time * 0.38f, v * 31.0f + worldZ * 11.0f, p.saltNoise2);
+        sourceNoise = 0.70f + 0.48f * fbm2(worldX * 13.0f + worldZ * 3.0f + time * 0.38f, v * 31.0f + worldZ * 11.0f, p.saltNoise2);
     }
     const float pyroGain = sid == 2 ? 0.42f : (sid == 1 ? 1.34f : 1.0f);
     const float fuelGain = sid == 2 ? 0.46f : (sid == 1 ? 1.18f : 1.0f);
@@ -271,14 +273,23 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
 template <bool B>
 __global__ __launch_bounds__(256) void k_advect_react(Scalars<B> out, Scalars<B> in, const float* __restrict__ U, const float* __restrict__ V,
                                                       const float* __restrict__ W, DevParams p, double* __restrict__ partial) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const uint3 blk = xcd_block();
+    const int x = blk.x * blockDim.x + threadIdx.x;
+    const int y = blk.y * blockDim.y + threadIdx.y;
+    const int z = blk.z * blockDim.z + threadIdx.z;
     float release = 0.0f;
     if (x < p.nx && y < p.ny && z < p.nz) release = react_cell<B>(out, in, U, V, W, p, x, y, z);
     const double t = block_sum_256(static_cast<double>(release));
     if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0)
         partial[(static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+}
+
+// Advances the device clock by one step (FP32 adds, as the host mirror does).
+__global__ void k_advance_clock(float* clock, float dt) {
+    if (threadIdx.x == 0) {
+        clock[0] += dt;
+        clock[1] += dt;
+    }
 }
 
 // Sums the react partials in a fixed order: acc[0] = step total, acc[1] += it.
@@ -297,9 +308,10 @@ template <bool B>
 __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, Scalars<B> S,
                                                        const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                        DevParams p) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const uint3 blk = xcd_block();
+    const int x = blk.x * blockDim.x + threadIdx.x;
+    const int y = blk.y * blockDim.y + threadIdx.y;
+    const int z = blk.z * blockDim.z + threadIdx.z;
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
     const size_t idx = sidx(p, x, y, z);
     const int sid = p.sceneId;
@@ -320,7 +332,7 @@ __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, f
     const float curlY = (uR - uL) - (wF - wB);
     const float curlZ = (vU - vD) - (uR - uL);
     const float curlMag = fabsf(curlX) + fabsf(curlY) + fabsf(curlZ);
-    const float t = p.time;
+    const float t = p.clock[0];
     const float noiseA = fbm2(u * 9.0f + w * 3.7f + t * 0.22f, v * 13.0f - t * 0.31f, p.saltNoise2) - 0.5f;
     const float noiseB = fbm2(w * 10.0f - t * 0.19f, u * 8.0f + v * 5.5f + t * 0.27f, p.saltNoise2) - 0.5f;
     const float thermal = sstep(0.04f, 4.20f, h + s * 0.28f);
@@ -367,9 +379,10 @@ __global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, flo
                                                      const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                      const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
                                                      float* __restrict__ D, float* __restrict__ P, DevParams p) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const uint3 blk = xcd_block();
+    const int x = blk.x * blockDim.x + threadIdx.x;
+    const int y = blk.y * blockDim.y + threadIdx.y;
+    const int z = blk.z * blockDim.z + threadIdx.z;
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
     const float uL = face_u(U, cu, p, x, y, z), uR = face_u(U, cu, p, x + 1, y, z);
     const float vD = face_v(V, cv, p, x, y, z), vU = face_v(V, cv, p, x, y + 1, z);
@@ -390,9 +403,10 @@ __global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, flo
 // One red/black half-sweep, in place: the updated colour (x+y+z)&1 == parity
 // reads only the other colour, so every cell of a half-sweep is independent.
 __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const float* __restrict__ D, int parity, float omega, int nx, int ny, int nz) {
-    const int tx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const uint3 blk = xcd_block();
+    const int tx = blk.x * blockDim.x + threadIdx.x;
+    const int y = blk.y * blockDim.y + threadIdx.y;
+    const int z = blk.z * blockDim.z + threadIdx.z;
     const int x = 2 * tx + ((parity ^ y ^ z) & 1);
     if (x >= nx || y >= ny || z >= nz) return;
     const size_t row = static_cast<size_t>(nx);
@@ -409,12 +423,109 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
     P[idx] = pc + omega * (projected - pc);
 }
 
+// Temporally blocked red-black SOR. A workgroup owns a T^3 tile, loads p on
+// the tile plus a K-cell halo (edge E = T + 2K) into LDS, and runs `count`
+// <= K consecutive half-sweeps there in place. After half-sweep s only cells
+// at least s+1 from an interior-facing face of the loaded box have correct
+// inputs, so the computed box shrinks by one per half-sweep (faces on the
+// domain boundary do not shrink: their neighbours are the cells themselves, as
+// in the reference's clamped indices). The interior T^3 is exact after K
+// half-sweeps and is the only part written, to a second pressure buffer
+// (tiles overlap through their halos, so a launch must not overwrite its own
+// input). Each cell update is the same FP32 expression, in the same order, as
+// k_sor_half, so the result is bit-identical to 2N plain half-sweeps.
+//
+// Thread layout (E^2 threads): for loads, thread t owns LDS column
+// (x = t % E, y = t / E) and walks z, so addresses only increment. For the
+// half-sweeps, thread t owns the colour cell pair k = t % (E/2) of row
+// y = (t / (E/2)) % E and every other z plane starting at z = t / (E^2/2);
+// along its planes the colour parity of its x position is fixed, so the wall
+// clamps in x and y become fixed LDS offsets (offset 0 reads the cell itself).
+// d is read from global memory (L2) inside the sweep, which keeps the kernel
+// at ~31 VGPRs and 55 KB of LDS so that two workgroups share a CU.
+template <int T, int K>
+__global__ __launch_bounds__((T + 2 * K) * (T + 2 * K)) void k_sor_tb(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D,
+                                                                     float omega, int h0, int count, int nx, int ny, int nz) {
+    constexpr int E = T + 2 * K;
+    constexpr int NT = E * E;  // threads per workgroup
+    constexpr int HX = E / 2;  // cells of one colour per LDS row
+    static_assert(E % 2 == 0 && NT % 64 == 0, "tile edge plus halo must be even and E^2 a multiple of 64");
+    __shared__ float sp[E * E * E];
+    const int tid = threadIdx.x;
+    const uint3 blk = xcd_block();
+    const int ox = blk.x * T, oy = blk.y * T, oz = blk.z * T;
+    const int bx = ox - K, by = oy - K, bz = oz - K;  // global coordinate of local 0
+    const size_t row = static_cast<size_t>(nx), plane = row * ny;
+    {
+        const int gx = bx + tid % E, gy = by + tid / E;
+        const bool colOk = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        float v[E];
+        const float* src = P + (colOk ? gy * row + gx : 0);
+#pragma unroll
+        for (int lz = 0; lz < E; ++lz) {
+            const int gz = bz + lz;
+            v[lz] = (colOk && gz >= 0 && gz < nz) ? src[gz * plane] : 0.0f;
+        }
+#pragma unroll
+        for (int lz = 0; lz < E; ++lz) sp[lz * NT + tid] = v[lz];
+    }
+    __syncthreads();
+    // local index range of real cells
+    const int rx0 = max(0, -bx), ry0 = max(0, -by), rz0 = max(0, -bz);
+    const int rx1 = min(E, nx - bx), ry1 = min(E, ny - by), rz1 = min(E, nz - bz);
+    const int k = tid % HX, ly = (tid / HX) % E, zg = tid / (HX * E);
+    const int gy = by + ly;
+    const int oD = gy > 0 ? -E : 0, oU = gy < ny - 1 ? E : 0;
+    for (int s = 0; s < count; ++s) {
+        const int parity = (h0 + s) & 1;
+        const int ax = bx < 0 ? rx0 : s + 1, ay = by < 0 ? ry0 : s + 1, az = bz < 0 ? rz0 : s + 1;
+        const int ex = ox + T + K > nx ? rx1 : E - 1 - s;
+        const int ey = oy + T + K > ny ? ry1 : E - 1 - s;
+        const int ez = oz + T + K > nz ? rz1 : E - 1 - s;
+        const int lx = 2 * k + ((parity ^ (bx + gy + bz + zg)) & 1);
+        if (lx >= ax && lx < ex && ly >= ay && ly < ey) {
+            const int gx = bx + lx;
+            const int oL = gx > 0 ? -1 : 0, oR = gx < nx - 1 ? 1 : 0;
+            const int lz0 = az + ((az ^ zg) & 1);
+            int i = (lz0 * E + ly) * E + lx;
+            const float* dg = D + (static_cast<size_t>(bz + lz0) * plane + gy * row + gx);
+#pragma unroll 2
+            for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, dg += 2 * plane) {
+                const int gz = bz + lz;
+                const float dv = *dg;
+                const float pc = sp[i];
+                const float pL = sp[i + oL];
+                const float pR = sp[i + oR];
+                const float pD = sp[i + oD];
+                const float pU = sp[i + oU];
+                const float pB = sp[i + (gz > 0 ? -NT : 0)];
+                const float pF = sp[i + (gz < nz - 1 ? NT : 0)];
+                const float projected = (dv + pL + pR + pD + pU + pB + pF) * (1.0f / 6.0f);
+                sp[i] = pc + omega * (projected - pc);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < T * T) {
+        const int tx = tid % T, ty = tid / T;
+        const int gx = ox + tx, gy2 = oy + ty;
+        if (gx < nx && gy2 < ny) {
+            float* dst = Pout + gy2 * row + gx;
+            const int src = (K * E + ty + K) * E + tx + K;
+#pragma unroll
+            for (int tz = 0; tz < T; ++tz)
+                if (oz + tz < nz) dst[(oz + tz) * plane] = sp[src + tz * NT];
+        }
+    }
+}
+
 // ----------------------------------------------------------- K13-K14 ----
 __global__ __launch_bounds__(256) void k_project(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, const float* __restrict__ P,
                                                  DevParams p) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const uint3 blk = xcd_block();
+    const int x = blk.x * blockDim.x + threadIdx.x;
+    const int y = blk.y * blockDim.y + threadIdx.y;
+    const int z = blk.z * blockDim.z + threadIdx.z;
     if (x > p.nx || y > p.ny || z > p.nz) return;
     if (y < p.ny && z < p.nz) {
         const size_t f = uidx(p, x, y, z);
@@ -436,9 +547,10 @@ __global__ __launch_bounds__(256) void k_project(float* __restrict__ U, float* _
 // -------------------------------------------------------------- reset ----
 template <bool B>
 __global__ __launch_bounds__(256) void k_reset(Scalars<B> S, float* __restrict__ P, float* __restrict__ D, DevParams p) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z * blockDim.z + threadIdx.z;
+    const uint3 blk = xcd_block();
+    const int x = blk.x * blockDim.x + threadIdx.x;
+    const int y = blk.y * blockDim.y + threadIdx.y;
+    const int z = blk.z * blockDim.z + threadIdx.z;
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
     const size_t idx = sidx(p, x, y, z);
     const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
@@ -450,7 +562,7 @@ __global__ __launch_bounds__(256) void k_reset(Scalars<B> S, float* __restrict__
     if (material != 0.0f) {
         const int sid = p.sceneId;
         const float chunk = fbm2(wx * 18.0f + wz * 2.0f, wz * 22.0f - wx * 5.0f, p.saltNoise2);
-        const float bed = bed_source(material, wx, wz, v, p);
+        const float bed = bed_source(material, wx, wz, v, p, 0.0f);
         const float charScale = sid == 2 ? 0.004f : (sid == 1 ? 1.18f : 1.0f);
         const float sootScale = sid == 2 ? 0.004f : (sid == 1 ? 0.92f : 1.0f);
         heat = bed * (sid == 2 ? 1.34f : (sid == 1 ? 0.78f + chunk * 0.26f : 0.86f + chunk * 0.24f));
@@ -597,6 +709,11 @@ hipError_t launch_reset(const DeviceScene& s, const DevParams& p, hipStream_t st
     return hipGetLastError();
 }
 
+hipError_t launch_advance_clock(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    k_advance_clock<<<1, 64, 0, st>>>(s.clock, p.dt);
+    return hipGetLastError();
+}
+
 hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st) {
     const dim3 g = cell_grid(s.nx + 1, s.ny + 1, s.nz + 1), b = cell_block();
     k_advect_velocity<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, p);
@@ -626,14 +743,54 @@ hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st) {
     return hipGetLastError();
 }
 
+int sor_depth(int sorKernel) {
+    switch (sorKernel) {
+        case 1: return 2;  // T20 K2
+        case 2: return 4;  // T16 K4
+        case 3: return 4;  // T8 K4
+        case 4: return 6;  // T12 K6
+        case 5: return 8;  // T8 K8
+        case 6: return 3;  // T18 K3
+        default: return 1;
+    }
+}
+
+template <int T, int K>
+static void launch_sor_tb(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    const dim3 g((s.nx + T - 1) / T, (s.ny + T - 1) / T, (s.nz + T - 1) / T);
+    const int half = 2 * p.iterations;
+    constexpr int E = T + 2 * K;
+    // Tiles read the halo of their neighbours, so a launch must not overwrite
+    // its own input: each launch reads one pressure buffer and writes the other.
+    for (int h0 = 0; h0 < half; h0 += K) {
+        k_sor_tb<T, K><<<g, E * E, 0, st>>>(s.p, s.p2, s.d, p.omega, h0, min(K, half - h0), s.nx, s.ny, s.nz);
+        std::swap(s.p, s.p2);
+    }
+}
+
 hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
-    const dim3 b = cell_block();
-    const dim3 g(((s.nx + 1) / 2 + b.x - 1) / b.x, (s.ny + b.y - 1) / b.y, (s.nz + b.z - 1) / b.z);
-    for (int i = 0; i < p.iterations; ++i) {
-        k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 0, p.omega, s.nx, s.ny, s.nz);
-        k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 1, p.omega, s.nx, s.ny, s.nz);
+    switch (s.sorKernel) {
+        case 1: launch_sor_tb<20, 2>(s, p, st); break;
+        case 2: launch_sor_tb<16, 4>(s, p, st); break;
+        case 3: launch_sor_tb<8, 4>(s, p, st); break;
+        case 4: launch_sor_tb<12, 6>(s, p, st); break;
+        case 5: launch_sor_tb<8, 8>(s, p, st); break;
+        case 6: launch_sor_tb<18, 3>(s, p, st); break;
+        default: {
+            const dim3 b = cell_block();
+            const dim3 g(((s.nx + 1) / 2 + b.x - 1) / b.x, (s.ny + b.y - 1) / b.y, (s.nz + b.z - 1) / b.z);
+            for (int i = 0; i < p.iterations; ++i) {
+                k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 0, p.omega, s.nx, s.ny, s.nz);
+                k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 1, p.omega, s.nx, s.ny, s.nz);
+            }
+        }
     }
     return hipGetLastError();
+}
+
+int pressure_launch_count(const DeviceScene& s, int iterations) {
+    const int k = sor_depth(s.sorKernel);
+    return (2 * iterations + k - 1) / k;
 }
 
 hipError_t launch_project(DeviceScene& s, const DevParams& p, hipStream_t st) {

@@ -30,6 +30,7 @@ STAGE_ADVECT_VELOCITY, STAGE_ADVECT_REACT, STAGE_FORCE, STAGE_PRESSURE, STAGE_PR
 STAGE_NAMES = ["advect_velocity", "advect_react", "force", "pressure", "project", "light"]
 
 STORAGE_FP32, STORAGE_BF16 = 0, 1
+SOR_KERNELS = ["half_sweep", "tb_t20_k2", "tb_t16_k4", "tb_t8_k4", "tb_t12_k6", "tb_t16_k4_dl", "tb_t18_k3"]
 SCENE_ROOM_TRAY, SCENE_CAMPFIRE, SCENE_GAS_BURNER, SCENE_EMITTERS_ONLY = 0, 1, 2, 3
 MAX_EMITTERS = 16
 
@@ -93,6 +94,14 @@ class Metrics(Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Tuning(Structure):
+    _fields_ = [("sor_kernel", c_int32), ("use_graph", c_int32)]
+
+
+class StageTimes(Structure):
+    _fields_ = [("stage_ms", c_double * 6), ("total_ms", c_double), ("steps", c_int32), ("pressure_launches", c_int32)]
 
 
 def default_params(**overrides) -> Params:
@@ -190,6 +199,10 @@ def load_library(path: str = LIB_PATH):
         "fs_last_error": (c_char_p, [c_void_p]),
         "fs_step": (c_int32, [c_void_p, c_int32]),
         "fs_run_stage": (c_int32, [c_void_p, c_int32]),
+        "fs_step_timed": (c_int32, [c_void_p, c_int32, P(StageTimes), P(c_double)]),
+        "fs_set_tuning": (c_int32, [c_void_p, P(Tuning)]),
+        "fs_get_tuning": (c_int32, [c_void_p, P(Tuning)]),
+        "fs_step_latency": (c_int32, [c_void_p, c_int32, P(c_double)]),
         "fs_synchronize": (c_int32, [c_void_p]),
         "fs_set_time": (c_int32, [c_void_p, c_float, c_float]),
         "fs_get_time": (c_int32, [c_void_p, P(c_float), P(c_float), P(c_int32)]),
@@ -276,6 +289,39 @@ class Scene:
 
     def step(self, n: int = 1):
         self._check(self.lib.fs_step(self.h, n), "fs_step")
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self._check(self.lib.fs_get_tuning(self.h, byref(t)), "fs_get_tuning")
+        return t
+
+    def set_tuning(self, sor_kernel=None, use_graph=None):
+        t = self.tuning()
+        if sor_kernel is not None:
+            t.sor_kernel = sor_kernel
+        if use_graph is not None:
+            t.use_graph = int(bool(use_graph))
+        self._check(self.lib.fs_set_tuning(self.h, byref(t)), "fs_set_tuning")
+
+    def set_sor_kernel(self, kernel: int):
+        """Selects the pressure-solve kernel (enum fs_sor_kernel); results are bit-identical."""
+        self.set_tuning(sor_kernel=kernel)
+
+    def sor_kernel(self) -> int:
+        return self.tuning().sor_kernel
+
+    def step_latency(self, n: int) -> np.ndarray:
+        """Runs n steps on the production path; returns per-step GPU ms."""
+        out = np.zeros(max(1, n), dtype=np.float64)
+        self._check(self.lib.fs_step_latency(self.h, n, out.ctypes.data_as(POINTER(c_double))), "fs_step_latency")
+        return out[:n]
+
+    def step_timed(self, n: int):
+        """Runs n steps with per-pass HIP event timing; returns (StageTimes, per-step ms)."""
+        st = StageTimes()
+        per = np.zeros(max(1, n), dtype=np.float64)
+        self._check(self.lib.fs_step_timed(self.h, n, byref(st), per.ctypes.data_as(POINTER(c_double))), "fs_step_timed")
+        return st, per[:n]
 
     def run_stage(self, stage: int):
         self._check(self.lib.fs_run_stage(self.h, stage), "fs_run_stage")
