@@ -190,14 +190,14 @@ typedef struct fs_stage_times {
 /* Pressure-solve kernels. All give bit-identical results (same per-cell FP32
  * expression in the same red/black order); they differ in how many
  * half-sweeps one launch performs from an LDS-resident tile of p (d read from
- * global memory unless noted). */
+ * global memory). */
 enum fs_sor_kernel {
     FS_SOR_HALF_SWEEP = 0,   /* one launch per half-sweep, as the reference (:3280-3284) */
     FS_SOR_TB_T20_K2 = 1,    /* 20^3 tiles, 2 half-sweeps per launch */
     FS_SOR_TB_T16_K4 = 2,    /* 16^3 tiles, 4 half-sweeps per launch (default) */
     FS_SOR_TB_T8_K4 = 3,     /* 8^3 tiles, 4 half-sweeps per launch */
     FS_SOR_TB_T12_K6 = 4,    /* 12^3 tiles, 6 half-sweeps per launch */
-    FS_SOR_TB_T16_K4_DL = 5, /* 16^3 tiles, 4 half-sweeps, d also staged in LDS */
+    FS_SOR_TB_T8_K8 = 5,     /* 8^3 tiles, 8 half-sweeps per launch */
     FS_SOR_TB_T18_K3 = 6,    /* 18^3 tiles, 3 half-sweeps per launch */
     FS_SOR_KERNEL_COUNT = 7
 };
@@ -207,6 +207,12 @@ typedef struct fs_tuning {
     int32_t sor_kernel; /* enum fs_sor_kernel */
     int32_t use_graph;  /* 1: fs_step replays a captured HIP graph of the step (default); 0: direct launches */
 } fs_tuning;
+
+/* Result of fs_autotune: time of one pressure pass per SOR kernel. */
+typedef struct fs_autotune_report {
+    double sor_ms[FS_SOR_KERNEL_COUNT];
+    int32_t chosen; /* enum fs_sor_kernel now set on the scene */
+} fs_autotune_report;
 
 /* ---- host-only helpers (no device needed) ---- */
 int fs_abi_version(void);
@@ -228,9 +234,16 @@ int fs_destroy(fs_scene* scene);
 int fs_reset(fs_scene* scene);
 int fs_set_params(fs_scene* scene, const fs_params* params);
 int fs_get_params(const fs_scene* scene, fs_params* out);
+/* The descriptor the scene was created with. */
+int fs_get_desc(const fs_scene* scene, fs_scene_desc* out);
 int fs_set_emitters(fs_scene* scene, const fs_emitter* emitters, int32_t count);
 int fs_set_tuning(fs_scene* scene, const fs_tuning* tuning);
 int fs_get_tuning(const fs_scene* scene, fs_tuning* out);
+/* Times every SOR kernel on this scene's grid and current divergence (HIP
+ * events, `reps` pressure passes each), selects the fastest and restores the
+ * pressure field. The choice depends only on grid size and iteration count;
+ * all kernels give identical results. */
+int fs_autotune(fs_scene* scene, int32_t reps, fs_autotune_report* out);
 const char* fs_last_error(const fs_scene* scene);
 
 /* ---- stepping (asynchronous on the scene stream) ---- */

@@ -576,6 +576,19 @@ int fs_set_params(fs_scene* s, const fs_params* prm) {
     return FS_OK;
 }
 
+int fs_get_desc(const fs_scene* s, fs_scene_desc* out) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (!out) return global_fail(FS_ERR_INVALID_ARGUMENT, "null output");
+    out->nx = s->dims.nx;
+    out->ny = s->dims.ny;
+    out->nz = s->dims.nz;
+    out->frame_w = s->dims.fw;
+    out->frame_h = s->dims.fh;
+    out->storage = s->dims.bf16 ? FS_STORAGE_BF16 : FS_STORAGE_FP32;
+    out->device = s->device;
+    return FS_OK;
+}
+
 int fs_get_params(const fs_scene* s, fs_params* out) {
     if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
     if (!out) return global_fail(FS_ERR_INVALID_ARGUMENT, "null output");
@@ -612,6 +625,46 @@ int fs_get_tuning(const fs_scene* s, fs_tuning* out) {
     if (!out) return global_fail(FS_ERR_INVALID_ARGUMENT, "null output");
     out->sor_kernel = s->ds.sorKernel;
     out->use_graph = s->useGraph ? 1 : 0;
+    return FS_OK;
+}
+
+int fs_autotune(fs_scene* s, int32_t reps, fs_autotune_report* out) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (!out || reps < 1 || reps > 100) return s->fail(FS_ERR_INVALID_ARGUMENT, "null report or reps out of range [1, 100]");
+    if (int st = set_device(s)) return st;
+    drop_graphs(s);
+    const size_t bytes = s->dims.cells() * sizeof(float);
+    // keep the pressure field in the force scratch buffer while candidates run
+    hipError_t e = hipMemcpyAsync(s->ds.cu, s->ds.p, bytes, hipMemcpyDeviceToDevice, s->stream);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    const DevParams p = dev_params(*s);
+    const int previous = s->ds.sorKernel;
+    int best = previous;
+    double bestMs = 1e30;
+    for (int k = 0; e == hipSuccess && k < FS_SOR_KERNEL_COUNT; ++k) {
+        s->ds.sorKernel = k;
+        e = fs::launch_pressure(s->ds, p, s->stream);  // warm
+        if (e == hipSuccess) e = hipEventRecord(t0, s->stream);
+        for (int r = 0; e == hipSuccess && r < reps; ++r) e = fs::launch_pressure(s->ds, p, s->stream);
+        if (e == hipSuccess) e = hipEventRecord(t1, s->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        out->sor_ms[k] = ms / reps;
+        if (e == hipSuccess && out->sor_ms[k] < bestMs) {
+            bestMs = out->sor_ms[k];
+            best = k;
+        }
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    s->ds.sorKernel = e == hipSuccess ? best : previous;
+    if (e == hipSuccess) e = hipMemcpyAsync(s->ds.p, s->ds.cu, bytes, hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return s->hip_fail("fs_autotune", e);
+    out->chosen = best;
     return FS_OK;
 }
 

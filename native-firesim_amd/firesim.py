@@ -30,7 +30,7 @@ STAGE_ADVECT_VELOCITY, STAGE_ADVECT_REACT, STAGE_FORCE, STAGE_PRESSURE, STAGE_PR
 STAGE_NAMES = ["advect_velocity", "advect_react", "force", "pressure", "project", "light"]
 
 STORAGE_FP32, STORAGE_BF16 = 0, 1
-SOR_KERNELS = ["half_sweep", "tb_t20_k2", "tb_t16_k4", "tb_t8_k4", "tb_t12_k6", "tb_t16_k4_dl", "tb_t18_k3"]
+SOR_KERNELS = ["half_sweep", "tb_t20_k2", "tb_t16_k4", "tb_t8_k4", "tb_t12_k6", "tb_t8_k8", "tb_t18_k3"]
 SCENE_ROOM_TRAY, SCENE_CAMPFIRE, SCENE_GAS_BURNER, SCENE_EMITTERS_ONLY = 0, 1, 2, 3
 MAX_EMITTERS = 16
 
@@ -98,6 +98,10 @@ class Metrics(Structure):
 
 class Tuning(Structure):
     _fields_ = [("sor_kernel", c_int32), ("use_graph", c_int32)]
+
+
+class AutotuneReport(Structure):
+    _fields_ = [("sor_ms", c_double * 7), ("chosen", c_int32)]
 
 
 class StageTimes(Structure):
@@ -195,6 +199,7 @@ def load_library(path: str = LIB_PATH):
         "fs_reset": (c_int32, [c_void_p]),
         "fs_set_params": (c_int32, [c_void_p, P(Params)]),
         "fs_get_params": (c_int32, [c_void_p, P(Params)]),
+        "fs_get_desc": (c_int32, [c_void_p, P(SceneDesc)]),
         "fs_set_emitters": (c_int32, [c_void_p, P(Emitter), c_int32]),
         "fs_last_error": (c_char_p, [c_void_p]),
         "fs_step": (c_int32, [c_void_p, c_int32]),
@@ -203,6 +208,7 @@ def load_library(path: str = LIB_PATH):
         "fs_set_tuning": (c_int32, [c_void_p, P(Tuning)]),
         "fs_get_tuning": (c_int32, [c_void_p, P(Tuning)]),
         "fs_step_latency": (c_int32, [c_void_p, c_int32, P(c_double)]),
+        "fs_autotune": (c_int32, [c_void_p, c_int32, P(AutotuneReport)]),
         "fs_synchronize": (c_int32, [c_void_p]),
         "fs_set_time": (c_int32, [c_void_p, c_float, c_float]),
         "fs_get_time": (c_int32, [c_void_p, P(c_float), P(c_float), P(c_int32)]),
@@ -250,6 +256,22 @@ class Scene:
             raise FireSimError(f"fs_create failed ({st}): {self.lib.fs_last_global_error().decode()}")
         self.h = h
 
+    @classmethod
+    def attach(cls, handle) -> "Scene":
+        """Non-owning view of an existing fs_scene handle (for example the one
+        behind the process-wide fireCuda* interface); close() leaves it alive."""
+        if not handle:
+            raise FireSimError("attach: null scene handle")
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.h = c_void_p(handle)
+        self._owned = False
+        desc = SceneDesc()
+        self._check(self.lib.fs_get_desc(self.h, byref(desc)), "fs_get_desc")
+        self.nx, self.ny, self.nz = desc.nx, desc.ny, desc.nz
+        self.frame_w, self.frame_h = desc.frame_w, desc.frame_h
+        return self
+
     def _check(self, st: int, what: str):
         if st != FS_OK:
             msg = self.lib.fs_last_error(self.h)
@@ -257,7 +279,8 @@ class Scene:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.fs_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.lib.fs_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -309,6 +332,12 @@ class Scene:
 
     def sor_kernel(self) -> int:
         return self.tuning().sor_kernel
+
+    def autotune(self, reps: int = 2) -> dict:
+        """Times every SOR kernel on this scene and keeps the fastest."""
+        r = AutotuneReport()
+        self._check(self.lib.fs_autotune(self.h, reps, byref(r)), "fs_autotune")
+        return {"sor_ms": {SOR_KERNELS[i]: r.sor_ms[i] for i in range(len(SOR_KERNELS))}, "chosen": SOR_KERNELS[r.chosen]}
 
     def step_latency(self, n: int) -> np.ndarray:
         """Runs n steps on the production path; returns per-step GPU ms."""

@@ -34,3 +34,21 @@ def test_sor_variants_bit_identical(hip_lib, grid, iters):
         s.close()
     for kernel, r in enumerate(results[1:], start=1):
         assert np.array_equal(r, results[0]), firesim.SOR_KERNELS[kernel]
+
+
+def test_autotune_restores_pressure_and_picks_a_kernel(hip_lib):
+    grid = (64, 48, 40)
+    rng = np.random.default_rng(9)
+    s = firesim.Scene(*grid, 8, 8)
+    s.set_params(compare.emitter_params(sor_iterations=20, sor_omega=1.7))
+    d = rng.normal(size=(grid[2], grid[1], grid[0])).astype(np.float32)
+    p0 = rng.normal(size=d.shape).astype(np.float32)
+    s.write("divergence", d)
+    s.write("pressure", p0)
+    rep = s.autotune(reps=1)
+    assert rep["chosen"] in firesim.SOR_KERNELS
+    assert all(ms > 0 for ms in rep["sor_ms"].values())
+    assert np.array_equal(s.read("pressure"), p0)
+    assert np.array_equal(s.read("divergence"), d)
+    assert firesim.SOR_KERNELS[s.sor_kernel()] == rep["chosen"]
+    s.close()
