@@ -99,11 +99,17 @@ bool step_and_render(std::uint32_t* bgra, const FireSettings& settings, FireCuda
     if (render && d3d) return fail("D3D11 interop is not available in this build; render to a host buffer with fireCudaStepAndRender.");
     if (!render && !advance) return fail("Step-only path requires simulation advancement.");
     if (!apply(settings)) return false;
-    // The reference resets fields and clocks in the same call, then steps; the
-    // step that follows a reset therefore starts at time dt here (the
-    // reference keeps the pre-reset time for that one frame, :3136-3143).
-    if (settings.reset != 0) {
+    // In-call reset (:3136-3143, :3181, :3053-3055): the reference advances
+    // its clocks, captures the step time, then the reset zeroes the clocks.
+    // So the frame that resets still steps and renders at the pre-reset time
+    // plus dt, and reports time 0; the next frame steps at time dt.
+    const bool resetNow = settings.reset != 0;
+    if (resetNow) {
+        float t0 = 0.0f, rt0 = 0.0f;
+        int32_t frame = 0;
+        fs_get_time(g_scene, &t0, &rt0, &frame);
         if (fs_reset(g_scene) != FS_OK) return fail_fs("reset");
+        if (fs_set_time(g_scene, t0, rt0) != FS_OK) return fail_fs("reset clock");
         g_frameIndex = 0;
     }
     fs_stage_times times{};
@@ -122,6 +128,7 @@ bool step_and_render(std::uint32_t* bgra, const FireSettings& settings, FireCuda
         if (fs_render_bgra8(g_scene, bgra) != FS_OK) return fail_fs("render");
         renderMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+    if (resetNow && fs_set_time(g_scene, 0.0f, 0.0f) != FS_OK) return fail_fs("reset clock");
     if (metrics) {
         fs_metrics m{};
         if (fs_metrics_collect(g_scene, &m) != FS_OK) return fail_fs("metrics");

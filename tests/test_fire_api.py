@@ -101,9 +101,14 @@ def test_frames_equal_the_per_scene_abi(hip_lib):
             assert API.step_and_render(a, s, m if i == 5 else None), API.last_error()
             scene.set_params(fire_api.settings_to_params(s))
             if s.reset:
+                # the reference steps the reset frame at the pre-reset clock, then zeroes the clocks
+                t0, rt0, _ = scene.get_time()
                 scene.reset()
+                scene.set_time(t0, rt0)
             scene.step(1)
             b[:] = scene.render_bgra8()
+            if s.reset:
+                scene.set_time(0.0, 0.0)
             assert np.array_equal(a.reshape(64, 96), b), f"frame {i}"
         ref = scene.metrics()
         api_scene = firesim.Scene.attach(API.fireCudaScene())
@@ -111,6 +116,7 @@ def test_frames_equal_the_per_scene_abi(hip_lib):
         assert (m.gridX, m.gridY, m.gridZ) == (100, 108, 66)
         assert m.pressureIterations == 40 and m.invalidCells == 0
         assert m.frameIndex == 2  # counted from the reset at frame 3
+        assert m.timeSeconds == pytest.approx(2 / 60, rel=1e-6)  # clocks zeroed by the reset frame
         assert m.heatSum == pytest.approx(ref["heat_sum"], rel=1e-6)
         assert m.divergenceAfterL2 == pytest.approx(ref["divergence_after_l2"], rel=1e-5)
         assert m.gpuSolveMs > 0 and m.gpuProjectionMs > 0
