@@ -230,6 +230,10 @@ const char* fs_last_global_error(void);
 
 /* ---- scene lifecycle ---- */
 int fs_create(const fs_scene_desc* desc, fs_scene** out);
+/* fs_create in caller-owned device memory (256-byte aligned, at least
+ * fs_scene_bytes(desc) bytes), for callers that recycle memory across scenes;
+ * fs_destroy leaves that memory to the caller. */
+int fs_create_in(const fs_scene_desc* desc, void* device_memory, uint64_t bytes, fs_scene** out);
 int fs_destroy(fs_scene* scene);
 int fs_reset(fs_scene* scene);
 int fs_set_params(fs_scene* scene, const fs_params* params);
@@ -264,6 +268,11 @@ int fs_get_time(const fs_scene* scene, float* time, float* render_time, int32_t*
 /* ---- rendering (synchronous: returns when the host buffer is written) ---- */
 int fs_render_hdr(fs_scene* scene, float* rgba_host);        /* frame_w*frame_h*4 floats */
 int fs_render_bgra8(fs_scene* scene, uint32_t* bgra_host);   /* frame_w*frame_h pixels */
+/* fs_render_bgra8 without the wait: render, tonemap and the copy into
+ * bgra_host are enqueued on the scene stream; the frame is complete once
+ * fs_synchronize (or an event after this call) has passed. bgra_host should be
+ * pinned (hipHostMalloc) for the copy to overlap other work. */
+int fs_render_bgra8_async(fs_scene* scene, uint32_t* bgra_host);
 /* Tonemaps a caller HDR frame with the scene's exposure (K21 alone). */
 int fs_tonemap(fs_scene* scene, const float* rgba_host, uint32_t* bgra_host);
 
@@ -276,6 +285,118 @@ int fs_metrics_collect(fs_scene* scene, fs_metrics* out);
 /* ---- measurement support ---- */
 /* Device stream of the scene (hipStream_t) for event timing by the caller. */
 void* fs_stream(fs_scene* scene);
+
+/* ---- batches: many scenes, one launch per pass ----
+ * No counterpart in the reference, which has one simulation per process
+ * (src/fire_cuda.cu:116-195). A batch steps every member scene once per
+ * fs_batch_step with one persistent launch per pass over all members (grids
+ * of any size, any SOR iteration count, one storage mode per batch). Results
+ * equal stepping each scene alone, bit for bit. While a scene is a member,
+ * all of its work (including per-scene calls) is issued on the batch stream.
+ * Membership and parameters may change between steps. */
+typedef struct fs_batch fs_batch;
+
+#define FS_MAX_BATCH 512
+
+int fs_batch_create(int32_t device, fs_batch** out);
+/* Removes every member (they keep their state) and frees the batch. */
+int fs_batch_destroy(fs_batch* batch);
+int fs_batch_add(fs_batch* batch, fs_scene* scene);
+int fs_batch_remove(fs_batch* batch, fs_scene* scene);
+int fs_batch_count(const fs_batch* batch, int32_t* count);
+/* sor_kernel applies to every member; use_graph replays captured step graphs. */
+int fs_batch_set_tuning(fs_batch* batch, const fs_tuning* tuning);
+int fs_batch_get_tuning(const fs_batch* batch, fs_tuning* out);
+/* Times each SOR kernel on the current members (HIP events, `reps` pressure
+ * passes each), keeps the fastest and restores every member's pressure. */
+int fs_batch_autotune(fs_batch* batch, int32_t reps, fs_autotune_report* out);
+/* Advances every member by nsteps (asynchronous on the batch stream). */
+int fs_batch_step(fs_batch* batch, int32_t nsteps);
+/* fs_batch_step that blocks and returns each step's GPU time (HIP events). */
+int fs_batch_step_latency(fs_batch* batch, int32_t nsteps, double* step_ms);
+int fs_batch_synchronize(fs_batch* batch);
+const char* fs_batch_last_error(const fs_batch* batch);
+void* fs_batch_stream(fs_batch* batch);
+
+/* ---- serving: continuous batching of simulation requests ----
+ * Extends the reference's worker loop (runCudaWorker src/main.cpp:3426-3677:
+ * resident state, step/render cadence, frame publish) from one simulation to
+ * a stream of requests. Each scheduling round (fs_server_tick):
+ *   1. takes note of finished GPU work and retires requests that are done;
+ *   2. admits arrived requests in FIFO order while their device memory fits
+ *      the HBM budget and a batch slot is free (state stays resident);
+ *   3. advances every admitted request by one step in one batched launch per
+ *      pass, and enqueues a frame render for requests at their render cadence.
+ * Requests arriving while others run join at the next round, so their work
+ * overlaps on the device. A request's results are the same as running it alone
+ * (fs_batch guarantees bit-identical steps). The host keeps at most two rounds
+ * in flight. Step latency of a request's k-th step is the time from when it
+ * became due (arrival for the first step, completion of the previous step
+ * otherwise) to the host observing its completion. */
+typedef struct fs_server fs_server;
+
+typedef struct fs_server_desc {
+    int32_t device;
+    int32_t max_active;         /* concurrently admitted requests, [1, FS_MAX_BATCH] */
+    uint64_t hbm_budget_bytes;  /* device memory for resident requests (fs_scene_bytes each) */
+    int32_t sor_kernel;         /* enum fs_sor_kernel; -1: autotune on the first rounds' members */
+    int32_t use_graph;          /* replay captured batch step graphs */
+} fs_server_desc;
+
+typedef struct fs_request {
+    fs_scene_desc desc;         /* grid, frame size, storage (device is the server's) */
+    fs_params params;
+    fs_emitter emitters[FS_MAX_EMITTERS];
+    int32_t emitter_count;
+    int32_t steps;              /* steps to run, >= 1 */
+    int32_t render_every;       /* render a frame every k steps (0: never) */
+    int32_t keep_scene;         /* 1: keep the scene after the last step for fs_server_take_scene */
+    double arrival_s;           /* arrival on the server clock (seconds since the first tick); < 0: now */
+} fs_request;
+
+enum fs_request_state { FS_REQ_PENDING = 0, FS_REQ_WAITING = 1, FS_REQ_RUNNING = 2, FS_REQ_DONE = 3, FS_REQ_FAILED = 4 };
+
+typedef struct fs_request_status {
+    int32_t state;              /* enum fs_request_state */
+    int32_t steps_done;         /* steps whose completion has been observed */
+    int32_t frames;             /* frames rendered */
+    int32_t reserved;
+    double arrival_s, admitted_s, finished_s;
+    double step_ms_p50, step_ms_p99, step_ms_max;
+} fs_request_status;
+
+typedef struct fs_server_stats {
+    int64_t requests_submitted, requests_done, requests_failed;
+    int64_t steps;              /* scene-steps completed */
+    int64_t voxel_steps;        /* sum of cells over completed scene-steps */
+    int64_t rounds;             /* scheduling rounds that stepped at least one request */
+    int64_t frames;
+    int32_t max_active_seen;
+    int32_t sor_kernel;         /* SOR kernel in use */
+    uint64_t peak_resident_bytes;
+    double busy_s;              /* first arrival to last completion */
+    double step_ms_p50, step_ms_p99, step_ms_mean, step_ms_max;
+    double queue_ms_p50, queue_ms_p99;   /* arrival to admission */
+} fs_server_stats;
+
+int fs_server_create(const fs_server_desc* desc, fs_server** out);
+/* Destroys every request's scene and frees the server. */
+int fs_server_destroy(fs_server* server);
+int fs_server_submit(fs_server* server, const fs_request* request, uint64_t* id);
+/* One scheduling round; *active receives the number of requests stepped. */
+int fs_server_tick(fs_server* server, int32_t* active);
+/* Rounds until every submitted request is done (sleeps until the next arrival when idle). */
+int fs_server_run(fs_server* server);
+int fs_server_status(fs_server* server, uint64_t id, fs_request_status* out);
+/* Latest frame of a request (frame_w*frame_h BGRA pixels); FS_ERR_INVALID_ARGUMENT if none yet. */
+int fs_server_frame(fs_server* server, uint64_t id, uint32_t* bgra, int32_t* frame_step);
+/* Hands a finished request's scene (kept with keep_scene) to the caller, who then owns it. */
+int fs_server_take_scene(fs_server* server, uint64_t id, fs_scene** out);
+int fs_server_stats_get(fs_server* server, fs_server_stats* out);
+/* Every observed step latency in ms, in completion order; *count receives the total. */
+int fs_server_step_latencies(fs_server* server, double* out_ms, int64_t capacity, int64_t* count);
+double fs_server_clock(fs_server* server);
+const char* fs_server_last_error(const fs_server* server);
 
 #ifdef __cplusplus
 }

@@ -36,9 +36,101 @@ struct DeviceScene {
     int sorKernel = 2;           // enum fs_sor_kernel
 };
 
+// ------------------------------------------------------------ batches ----
+// A batch steps many scenes of different sizes with one launch per pass. The
+// launch is persistent: a fixed grid of workgroups walks the concatenated
+// block lists of all scenes (per-scene tile grids identical to the one-scene
+// launches, so every cell sees the same arithmetic and results equal the
+// one-scene path bit for bit). Scenes are ordered by SOR half-sweep count,
+// largest first, so the scenes still active in pressure launch j are a prefix.
+constexpr int kMaxBatch = 512;
+constexpr int kMaxSorLaunches = 4096;
+
+// One scene as the passes of one step see it (the buffer roles the one-scene
+// launchers would assign before the step).
+struct BatchScene {
+    DevParams p;
+    void* sIn[NSCALAR];   // scalars at step start
+    void* sOut[NSCALAR];  // advect-react output = scalars after the step
+    float *velA[3];       // u, v, w at step start; force output; projected
+    float *velB[3];       // advected velocity; force input
+    float *p0, *p1;       // pressure buffers in launch order (p0 zeroed by the force pass)
+    float* pFinal;        // buffer holding the solution after the pressure pass
+    float *d, *cu, *cv, *cw;
+    float4 *light, *lightNext;
+    float* shadow;
+    double* partials;
+    double* acc;
+    float* clock;
+    int halfSweeps;                 // 2 * SOR iterations
+    int cellG[3], faceG[3], sorG[3], lightG[3];  // per-pass block grids
+};
+
+struct BatchTable {
+    int count;
+    int sorDepth;  // half-sweeps per pressure launch (1: plain half-sweep kernel)
+    uint32_t cellStart[kMaxBatch + 1];  // prefix sums of per-scene block counts
+    uint32_t faceStart[kMaxBatch + 1];
+    uint32_t sorStart[kMaxBatch + 1];
+    uint32_t lightStart[kMaxBatch + 1];
+    uint16_t sorActive[kMaxSorLaunches];  // scenes taking part in pressure launch j
+    BatchScene scene[1];  // `count` entries follow
+};
+
+inline size_t batch_table_bytes(int count) { return sizeof(BatchTable) + sizeof(BatchScene) * (count > 0 ? count - 1 : 0); }
+
+// Persistent walk over the concatenated block lists of the first `count`
+// scenes (start[] = prefix sums of their block counts). The hardware hands
+// workgroup w to XCD w % 8; XCD i takes the i-th eighth of the logical blocks
+// and its workgroups stride through it, so the tiles an XCD works on are
+// neighbours that share its L2. f(scene, local block) runs with
+// workgroup-uniform arguments, so it may contain barriers.
+template <class F>
+__device__ __forceinline__ void batch_blocks(const uint32_t* __restrict__ start, int count, F&& f) {
+    if (count <= 0) return;
+    const uint32_t total = start[count];
+    const unsigned per = gridDim.x / kXcdCount;
+    const unsigned xcd = blockIdx.x % kXcdCount, slot = blockIdx.x / kXcdCount;
+    if (slot >= per) return;
+    const uint32_t c0 = static_cast<uint32_t>((static_cast<uint64_t>(total) * xcd) / kXcdCount);
+    const uint32_t c1 = static_cast<uint32_t>((static_cast<uint64_t>(total) * (xcd + 1)) / kXcdCount);
+    uint32_t b = c0 + slot;
+    if (b >= c1) return;
+    int lo = 0, hi = count - 1;  // scene of block b: last s with start[s] <= b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    int s = lo;
+    for (; b < c1; b += per) {
+        while (start[s + 1] <= b) ++s;
+        f(s, b - start[s]);
+    }
+}
+
+// Local block number -> 3D block coordinate of a per-scene grid g.
+__device__ __forceinline__ uint3 block_of(uint32_t local, const int* g) {
+    const uint32_t gx = static_cast<uint32_t>(g[0]), gy = static_cast<uint32_t>(g[1]);
+    return make_uint3(local % gx, (local / gx) % gy, local / (gx * gy));
+}
+
+hipError_t launch_batch_light(const BatchTable* table, bool bf16, int grid, hipStream_t st);
+
+// Launches one batched step: clock, the six passes. `grid` is the persistent
+// workgroup count (a multiple of 8); `sorLaunches` the pressure launch count.
+hipError_t launch_batch_step(const BatchTable* table, bool bf16, int sorKernel, int sorLaunches, int grid, hipStream_t st);
+// The pressure launches of a batched step alone (autotuning).
+hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sorLaunches, int grid, hipStream_t st);
+// Persistent grid size for this device (workgroups resident at once, a multiple of 16).
+int batch_grid(int device);
+
 // Grid and block shape of the cell-parallel passes (64 lanes along x).
 dim3 cell_block();
 dim3 cell_grid(int nx, int ny, int nz);
+dim3 light_block();
+dim3 light_grid(int lnx, int lny, int lnz);
+dim3 sor_grid(int sorKernel, int nx, int ny, int nz);
 
 hipError_t launch_reset(const DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_advance_clock(DeviceScene& s, const DevParams& p, hipStream_t st);
@@ -60,5 +152,6 @@ size_t react_partial_count(int nx, int ny, int nz);
 int pressure_launch_count(const DeviceScene& s, int iterations);
 // Half-sweeps one temporally blocked SOR launch performs (1 for the plain kernel).
 int sor_depth(int sorKernel);
+int sor_tile_edge(int sorKernel);
 
 }  // namespace fs

@@ -10,10 +10,8 @@ namespace fs {
 
 // ---------------------------------------------------------------- K17 ----
 template <bool B>
-__global__ __launch_bounds__(256) void k_light_build(float4* __restrict__ L, float* __restrict__ SH, Scalars<B> S, DevParams p) {
-    const int lx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ly = blockIdx.y * blockDim.y + threadIdx.y;
-    const int lz = blockIdx.z * blockDim.z + threadIdx.z;
+__device__ __forceinline__ void light_build_at(float4* __restrict__ L, float* __restrict__ SH, const Scalars<B>& S, const DevParams& p, int lx, int ly,
+                                               int lz) {
     if (lx >= p.lnx || ly >= p.lny || lz >= p.lnz) return;
     const float u = (static_cast<float>(lx) + 0.5f) / static_cast<float>(p.lnx);
     const float v = (static_cast<float>(ly) + 0.5f) / static_cast<float>(p.lny);
@@ -63,12 +61,14 @@ __global__ __launch_bounds__(256) void k_light_build(float4* __restrict__ L, flo
     SH[idx] = expf(-(optical * (0.72f + vertDepth * 0.42f) + radial * optical * 0.34f + progress * 0.035f));
 }
 
+template <bool B>
+__global__ __launch_bounds__(256) void k_light_build(float4* __restrict__ L, float* __restrict__ SH, Scalars<B> S, DevParams p) {
+    light_build_at<B>(L, SH, S, p, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z * blockDim.z + threadIdx.z);
+}
+
 // ---------------------------------------------------------------- K18 ----
-__global__ __launch_bounds__(256) void k_light_propagate(float4* __restrict__ out, const float4* __restrict__ in, const float* __restrict__ SH,
-                                                         DevParams p, float diffusion, float decay) {
-    const int lx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ly = blockIdx.y * blockDim.y + threadIdx.y;
-    const int lz = blockIdx.z * blockDim.z + threadIdx.z;
+__device__ __forceinline__ void light_propagate_at(float4* __restrict__ out, const float4* __restrict__ in, const float* __restrict__ SH,
+                                                   const DevParams& p, float diffusion, float decay, int lx, int ly, int lz) {
     if (lx >= p.lnx || ly >= p.lny || lz >= p.lnz) return;
     auto li = [&](int x, int y, int z) { return (static_cast<size_t>(z) * p.lny + y) * p.lnx + x; };
     const size_t idx = li(lx, ly, lz);
@@ -93,6 +93,12 @@ __global__ __launch_bounds__(256) void k_light_propagate(float4* __restrict__ ou
     const F3 contact = mul3(f3(1.0f, 0.40f, 0.12f), floorLift * lum3(direct) * 0.18f);
     const F3 next = add3(mul3(bounced, decay * (0.66f + shadow * 0.34f)), contact);
     out[idx] = make_float4(next.x, next.y, next.z, center.w);
+}
+
+__global__ __launch_bounds__(256) void k_light_propagate(float4* __restrict__ out, const float4* __restrict__ in, const float* __restrict__ SH,
+                                                         DevParams p, float diffusion, float decay) {
+    light_propagate_at(out, in, SH, p, diffusion, decay, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+                       blockIdx.z * blockDim.z + threadIdx.z);
 }
 
 // ------------------------------------------------------------- render ----
@@ -606,13 +612,49 @@ static Scalars<B> scal(void* const* f) {
     return s;
 }
 
+dim3 light_block() { return dim3(8, 8, 4); }
+dim3 light_grid(int lnx, int lny, int lnz) { return dim3((lnx + 7) / 8, (lny + 7) / 8, (lnz + 3) / 4); }
+
 hipError_t launch_light(DeviceScene& s, const DevParams& p, hipStream_t st) {
-    const dim3 b(8, 8, 4);
-    const dim3 g((s.lnx + 7) / 8, (s.lny + 7) / 8, (s.lnz + 3) / 4);
+    const dim3 b = light_block();
+    const dim3 g = light_grid(s.lnx, s.lny, s.lnz);
     if (s.bf16) k_light_build<true><<<g, b, 0, st>>>(s.light, s.shadow, scal<true>(s.s), p);
     else k_light_build<false><<<g, b, 0, st>>>(s.light, s.shadow, scal<false>(s.s), p);
     k_light_propagate<<<g, b, 0, st>>>(s.lightNext, s.light, s.shadow, p, 0.48f, 0.92f);
     k_light_propagate<<<g, b, 0, st>>>(s.light, s.lightNext, s.shadow, p, 0.34f, 0.96f);
+    return hipGetLastError();
+}
+
+template <bool B>
+__global__ __launch_bounds__(256) void kb_light_build(const BatchTable* __restrict__ T) {
+    batch_blocks(T->lightStart, T->count, [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const uint3 b = block_of(local, S.lightG);
+        Scalars<B> sc;
+#pragma unroll
+        for (int i = 0; i < NSCALAR; ++i) sc.f[i] = S.sOut[i];
+        light_build_at<B>(S.light, S.shadow, sc, S.p, b.x * blockDim.x + threadIdx.x, b.y * blockDim.y + threadIdx.y, b.z * blockDim.z + threadIdx.z);
+    });
+}
+
+// pass 0: light -> lightNext with (0.48, 0.92); pass 1: back with (0.34, 0.96).
+__global__ __launch_bounds__(256) void kb_light_propagate(const BatchTable* __restrict__ T, int pass) {
+    batch_blocks(T->lightStart, T->count, [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const uint3 b = block_of(local, S.lightG);
+        const int lx = b.x * blockDim.x + threadIdx.x, ly = b.y * blockDim.y + threadIdx.y, lz = b.z * blockDim.z + threadIdx.z;
+        if (pass == 0) light_propagate_at(S.lightNext, S.light, S.shadow, S.p, 0.48f, 0.92f, lx, ly, lz);
+        else light_propagate_at(S.light, S.lightNext, S.shadow, S.p, 0.34f, 0.96f, lx, ly, lz);
+    });
+}
+
+hipError_t launch_batch_light(const BatchTable* table, bool bf16, int grid, hipStream_t st) {
+    const dim3 b = light_block();
+    const int g = grid / 4 < 16 ? 16 : grid / 4 - (grid / 4) % 8;  // the light grid is 1/64 of the cells
+    if (bf16) kb_light_build<true><<<g, b, 0, st>>>(table);
+    else kb_light_build<false><<<g, b, 0, st>>>(table);
+    kb_light_propagate<<<g, b, 0, st>>>(table, 0);
+    kb_light_propagate<<<g, b, 0, st>>>(table, 1);
     return hipGetLastError();
 }
 

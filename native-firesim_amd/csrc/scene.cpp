@@ -15,11 +15,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/firesim.h"
-#include "fs_launch.hpp"
-
-using fs::DeviceScene;
-using fs::DevParams;
+#include "scene_impl.hpp"
 
 namespace {
 
@@ -88,16 +84,6 @@ struct Layout {
     size_t total = 0;
 };
 
-struct Dims {
-    int nx, ny, nz, lnx, lny, lnz, fw, fh;
-    bool bf16;
-    size_t cells() const { return static_cast<size_t>(nx) * ny * nz; }
-    size_t lcells() const { return static_cast<size_t>(lnx) * lny * lnz; }
-    size_t pixels() const { return static_cast<size_t>(fw) * fh; }
-    size_t partials() const {
-        return std::max(fs::react_partial_count(nx, ny, nz), static_cast<size_t>(fs::kMetricBlocks) * fs::kMetricSlots);
-    }
-};
 
 // Byte size of each sub-buffer, in allocation order.
 std::vector<size_t> buffer_sizes(const Dims& d) {
@@ -173,34 +159,6 @@ Dims dims_of(const fs_scene_desc& d) {
 
 }  // namespace
 
-struct fs_scene {
-    int device = 0;
-    Dims dims{};
-    hipStream_t stream = nullptr;
-    void* pool = nullptr;
-    size_t poolBytes = 0;
-    DeviceScene ds;
-    fs_params prm{};
-    std::vector<fs_emitter> emitters;
-    float time = 0.0f, renderTime = 0.0f;
-    int frameIndex = 0;
-    std::string lastError = "no error";
-    // Step graphs. Scalar fields and (for an odd number of SOR launches) the
-    // pressure buffers change places every step, so two one-step graphs are
-    // captured, from the buffer assignment before an even and an odd step;
-    // state[g] is the assignment graph g starts from.
-    bool useGraph = true;
-    bool graphsValid = false;
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    DeviceScene state[2];
-    int parity = 0;  // graph to run next
-
-    int fail(int status, const std::string& msg) {
-        lastError = msg;
-        return status;
-    }
-    int hip_fail(const char* what, hipError_t e) { return fail(FS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)); }
-};
 
 namespace {
 
@@ -416,6 +374,12 @@ bool narrow_field(const fs_scene* s, int f) {
 
 }  // namespace
 
+namespace fsi {
+DevParams dev_params(const fs_scene& sc) { return ::dev_params(sc); }
+void drop_graphs(fs_scene* s) { ::drop_graphs(s); }
+int global_fail(int status, const std::string& msg) { return ::global_fail(status, msg); }
+}  // namespace fsi
+
 // ====================================================================== ABI
 extern "C" {
 
@@ -471,15 +435,22 @@ int fs_scene_bytes(const fs_scene_desc* desc, uint64_t* bytes) {
 
 const char* fs_last_global_error(void) { return g_globalError.c_str(); }
 
-int fs_create(const fs_scene_desc* desc, fs_scene** out) {
+namespace {
+
+// fs_create / fs_create_in: `memory` is null (the scene allocates its pool)
+// or caller-owned device memory of at least total_bytes(dims) bytes.
+int create_scene(const fs_scene_desc* desc, void* memory, uint64_t memoryBytes, fs_scene** out) {
     std::string why;
     if (!out) return global_fail(FS_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
     if (!valid_desc(desc, why)) return global_fail(FS_ERR_INVALID_ARGUMENT, why);
+    const Dims dims = dims_of(*desc);
+    if (memory && (memoryBytes < total_bytes(dims) || (reinterpret_cast<uintptr_t>(memory) & 255u) != 0))
+        return global_fail(FS_ERR_INVALID_ARGUMENT, "scene memory must be 256-byte aligned and at least fs_scene_bytes long");
     fs_scene* s = new (std::nothrow) fs_scene();
     if (!s) return global_fail(FS_ERR_OUT_OF_MEMORY, "host allocation failed");
     s->device = desc->device;
-    s->dims = dims_of(*desc);
+    s->dims = dims;
     hipError_t e = hipSetDevice(s->device);
     if (e != hipSuccess) {
         delete s;
@@ -487,19 +458,25 @@ int fs_create(const fs_scene_desc* desc, fs_scene** out) {
     }
     const std::vector<size_t> sizes = buffer_sizes(s->dims);
     s->poolBytes = total_bytes(s->dims);
-    e = hipMalloc(&s->pool, s->poolBytes);
-    if (e != hipSuccess) {
-        delete s;
-        (void)hipGetLastError();
-        return global_fail(e == hipErrorOutOfMemory ? FS_ERR_OUT_OF_MEMORY : FS_ERR_HIP,
-                           std::string("hipMalloc scene: ") + hipGetErrorString(e));
+    if (memory) {
+        s->pool = memory;
+        s->ownsPool = false;
+    } else {
+        e = hipMalloc(&s->pool, s->poolBytes);
+        if (e != hipSuccess) {
+            delete s;
+            (void)hipGetLastError();
+            return global_fail(e == hipErrorOutOfMemory ? FS_ERR_OUT_OF_MEMORY : FS_ERR_HIP,
+                               std::string("hipMalloc scene: ") + hipGetErrorString(e));
+        }
     }
-    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    e = hipStreamCreateWithFlags(&s->ownStream, hipStreamNonBlocking);
     if (e != hipSuccess) {
-        (void)hipFree(s->pool);
+        if (s->ownsPool) (void)hipFree(s->pool);
         delete s;
         return global_fail(FS_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
+    s->stream = s->ownStream;
     char* base = static_cast<char*>(s->pool);
     size_t off = 0, k = 0;
     auto take = [&]() {
@@ -538,13 +515,23 @@ int fs_create(const fs_scene_desc* desc, fs_scene** out) {
     return FS_OK;
 }
 
+}  // namespace
+
+int fs_create(const fs_scene_desc* desc, fs_scene** out) { return create_scene(desc, nullptr, 0, out); }
+
+int fs_create_in(const fs_scene_desc* desc, void* device_memory, uint64_t bytes, fs_scene** out) {
+    if (!device_memory) return global_fail(FS_ERR_INVALID_ARGUMENT, "null scene memory");
+    return create_scene(desc, device_memory, bytes, out);
+}
+
 int fs_destroy(fs_scene* s) {
     if (!s) return FS_OK;
     (void)hipSetDevice(s->device);
+    if (s->batch) (void)fs_batch_remove(s->batch, s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graphs(s);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
-    if (s->pool) (void)hipFree(s->pool);
+    if (s->ownStream) (void)hipStreamDestroy(s->ownStream);
+    if (s->pool && s->ownsPool) (void)hipFree(s->pool);
     delete s;
     return FS_OK;
 }
@@ -819,6 +806,17 @@ int fs_render_hdr(fs_scene* s, float* rgba) {
     e = hipMemcpyAsync(rgba, s->ds.hdr, s->dims.pixels() * 16, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     return e == hipSuccess ? FS_OK : s->hip_fail("render readback", e);
+}
+
+int fs_render_bgra8_async(fs_scene* s, uint32_t* bgra) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (!bgra) return s->fail(FS_ERR_INVALID_ARGUMENT, "output pixel pointer was null");
+    if (int st = set_device(s)) return st;
+    const DevParams p = dev_params(*s);
+    hipError_t e = fs::launch_render(s->ds, p, s->stream);
+    if (e == hipSuccess) e = fs::launch_tonemap(s->ds, p, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(bgra, s->ds.bgra, s->dims.pixels() * 4, hipMemcpyDeviceToHost, s->stream);
+    return e == hipSuccess ? FS_OK : s->hip_fail("render launch", e);
 }
 
 int fs_render_bgra8(fs_scene* s, uint32_t* bgra) {

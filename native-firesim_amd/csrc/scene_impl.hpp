@@ -1,0 +1,68 @@
+// scene_impl.hpp - the scene handle behind include/firesim.h, shared by the
+// per-scene ABI (scene.cpp) and batches (batch.cpp). Not part of the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/firesim.h"
+#include "fs_launch.hpp"
+
+using fs::DeviceScene;
+using fs::DevParams;
+
+struct fs_batch;
+
+struct Dims {
+    int nx, ny, nz, lnx, lny, lnz, fw, fh;
+    bool bf16;
+    size_t cells() const { return static_cast<size_t>(nx) * ny * nz; }
+    size_t lcells() const { return static_cast<size_t>(lnx) * lny * lnz; }
+    size_t pixels() const { return static_cast<size_t>(fw) * fh; }
+    size_t partials() const {
+        return std::max(fs::react_partial_count(nx, ny, nz), static_cast<size_t>(fs::kMetricBlocks) * fs::kMetricSlots);
+    }
+};
+
+struct fs_scene {
+    int device = 0;
+    Dims dims{};
+    hipStream_t stream = nullptr;     // stream all of the scene's work is issued on
+    hipStream_t ownStream = nullptr;  // the scene's own stream (stream == ownStream outside a batch)
+    fs_batch* batch = nullptr;        // batch the scene belongs to, if any
+    void* pool = nullptr;
+    size_t poolBytes = 0;
+    bool ownsPool = true;  // false: fs_create_in memory, left to the caller
+    DeviceScene ds;
+    fs_params prm{};
+    std::vector<fs_emitter> emitters;
+    float time = 0.0f, renderTime = 0.0f;
+    int frameIndex = 0;
+    std::string lastError = "no error";
+    // Step graphs. Scalar fields and (for an odd number of SOR launches) the
+    // pressure buffers change places every step, so two one-step graphs are
+    // captured, from the buffer assignment before an even and an odd step;
+    // state[g] is the assignment graph g starts from.
+    bool useGraph = true;
+    bool graphsValid = false;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    DeviceScene state[2];
+    int parity = 0;  // graph to run next
+
+    int fail(int status, const std::string& msg) {
+        lastError = msg;
+        return status;
+    }
+    int hip_fail(const char* what, hipError_t e) { return fail(FS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)); }
+};
+
+namespace fsi {
+// Kernel parameter block of a scene's current parameters and emitters.
+DevParams dev_params(const fs_scene& sc);
+// Releases the scene's captured step graphs (they are recaptured on demand).
+void drop_graphs(fs_scene* s);
+// Sets the thread's handle-less error message; returns status.
+int global_fail(int status, const std::string& msg);
+}  // namespace fsi

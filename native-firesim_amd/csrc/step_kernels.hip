@@ -49,13 +49,10 @@ __device__ __forceinline__ double block_sum_256(double v) {
 }
 
 // ------------------------------------------------------------ K5-K7 ----
-__global__ __launch_bounds__(256) void k_advect_velocity(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
-                                                         const float* __restrict__ U, const float* __restrict__ V,
-                                                         const float* __restrict__ W, DevParams p) {
-    const uint3 blk = xcd_block();
-    const int x = blk.x * blockDim.x + threadIdx.x;
-    const int y = blk.y * blockDim.y + threadIdx.y;
-    const int z = blk.z * blockDim.z + threadIdx.z;
+// Thread (x, y, z) of a face grid of (nx+1, ny+1, nz+1) writes u, v and w faces.
+__device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                                                   const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                                   const DevParams& p, int x, int y, int z) {
     if (x > p.nx || y > p.ny || z > p.nz) return;
     const float fnx = static_cast<float>(p.nx), fny = static_cast<float>(p.ny), fnz = static_cast<float>(p.nz);
     if (y < p.ny && z < p.nz) {
@@ -88,6 +85,18 @@ __global__ __launch_bounds__(256) void k_advect_velocity(float* __restrict__ uo,
         }
         wo[widx(p, x, y, z)] = out;
     }
+}
+
+__device__ __forceinline__ int3 cell_of(uint3 blk) {
+    return make_int3(static_cast<int>(blk.x * blockDim.x + threadIdx.x), static_cast<int>(blk.y * blockDim.y + threadIdx.y),
+                     static_cast<int>(blk.z * blockDim.z + threadIdx.z));
+}
+
+__global__ __launch_bounds__(256) void k_advect_velocity(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                                                         const float* __restrict__ U, const float* __restrict__ V,
+                                                         const float* __restrict__ W, DevParams p) {
+    const int3 c = cell_of(xcd_block());
+    advect_velocity_at(uo, vo, wo, U, V, W, p, c.x, c.y, c.z);
 }
 
 // --------------------------------------------------------------- K8 ----
@@ -274,14 +283,12 @@ template <bool B>
 __global__ __launch_bounds__(256) void k_advect_react(Scalars<B> out, Scalars<B> in, const float* __restrict__ U, const float* __restrict__ V,
                                                       const float* __restrict__ W, DevParams p, double* __restrict__ partial) {
     const uint3 blk = xcd_block();
-    const int x = blk.x * blockDim.x + threadIdx.x;
-    const int y = blk.y * blockDim.y + threadIdx.y;
-    const int z = blk.z * blockDim.z + threadIdx.z;
+    const int3 c = cell_of(blk);
     float release = 0.0f;
-    if (x < p.nx && y < p.ny && z < p.nz) release = react_cell<B>(out, in, U, V, W, p, x, y, z);
+    if (c.x < p.nx && c.y < p.ny && c.z < p.nz) release = react_cell<B>(out, in, U, V, W, p, c.x, c.y, c.z);
     const double t = block_sum_256(static_cast<double>(release));
-    if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0)
-        partial[(static_cast<size_t>(blockIdx.z) * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+    // one partial per logical tile, so the batched pass fills the same slots
+    if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) partial[(static_cast<size_t>(blk.z) * gridDim.y + blk.y) * gridDim.x + blk.x] = t;
 }
 
 // Advances the device clock by one step (FP32 adds, as the host mirror does).
@@ -305,13 +312,9 @@ __global__ __launch_bounds__(256) void k_finalize_release(const double* __restri
 
 // ---------------------------------------------------------- K9-K11 ----
 template <bool B>
-__global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, Scalars<B> S,
-                                                       const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                                                       DevParams p) {
-    const uint3 blk = xcd_block();
-    const int x = blk.x * blockDim.x + threadIdx.x;
-    const int y = blk.y * blockDim.y + threadIdx.y;
-    const int z = blk.z * blockDim.z + threadIdx.z;
+__device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, const Scalars<B>& S,
+                                                 const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                                 const DevParams& p, int x, int y, int z) {
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
     const size_t idx = sidx(p, x, y, z);
     const int sid = p.sceneId;
@@ -351,6 +354,14 @@ __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, f
     cw[idx] = p.dt * (p.windZ * windProfile + (gradX * 1.36f + noiseB * (0.20f + upperDrag * 0.16f) + curlZ * vortexGain) * curlGain);
 }
 
+template <bool B>
+__global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, Scalars<B> S,
+                                                       const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                                       DevParams p) {
+    const int3 c = cell_of(xcd_block());
+    force_contrib_at<B>(cu, cv, cw, S, U, V, W, p, c.x, c.y, c.z);
+}
+
 // New face values after the gathered force and the boundary clamp
 // (velocityBoundaryKernel :1261-1291). Both cells sharing a face evaluate the
 // same expression, so the face is bit-identical from either side.
@@ -375,14 +386,10 @@ __device__ __forceinline__ float face_w(const float* W, const float* cw, const D
     return fminf(4.0f, fmaxf(-4.0f, val));
 }
 
-__global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
-                                                     const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                                                     const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
-                                                     float* __restrict__ D, float* __restrict__ P, DevParams p) {
-    const uint3 blk = xcd_block();
-    const int x = blk.x * blockDim.x + threadIdx.x;
-    const int y = blk.y * blockDim.y + threadIdx.y;
-    const int z = blk.z * blockDim.z + threadIdx.z;
+__device__ __forceinline__ void force_apply_at(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                                               const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                               const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
+                                               float* __restrict__ D, float* __restrict__ P, const DevParams& p, int x, int y, int z) {
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
     const float uL = face_u(U, cu, p, x, y, z), uR = face_u(U, cu, p, x + 1, y, z);
     const float vD = face_v(V, cv, p, x, y, z), vU = face_v(V, cv, p, x, y + 1, z);
@@ -399,14 +406,20 @@ __global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, flo
     P[idx] = 0.0f;
 }
 
+__global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                                                     const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                                     const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
+                                                     float* __restrict__ D, float* __restrict__ P, DevParams p) {
+    const int3 c = cell_of(xcd_block());
+    force_apply_at(uo, vo, wo, U, V, W, cu, cv, cw, D, P, p, c.x, c.y, c.z);
+}
+
 // -------------------------------------------------------------- K12 ----
 // One red/black half-sweep, in place: the updated colour (x+y+z)&1 == parity
 // reads only the other colour, so every cell of a half-sweep is independent.
-__global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const float* __restrict__ D, int parity, float omega, int nx, int ny, int nz) {
-    const uint3 blk = xcd_block();
-    const int tx = blk.x * blockDim.x + threadIdx.x;
-    const int y = blk.y * blockDim.y + threadIdx.y;
-    const int z = blk.z * blockDim.z + threadIdx.z;
+// Thread (tx, y, z) of a ((nx+1)/2, ny, nz) grid updates cell x = 2tx + colour.
+__device__ __forceinline__ void sor_half_at(float* __restrict__ P, const float* __restrict__ D, int parity, float omega, int nx, int ny, int nz,
+                                            int tx, int y, int z) {
     const int x = 2 * tx + ((parity ^ y ^ z) & 1);
     if (x >= nx || y >= ny || z >= nz) return;
     const size_t row = static_cast<size_t>(nx);
@@ -421,6 +434,11 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
     const float pF = z < nz - 1 ? P[idx + plane] : pc;
     const float projected = (D[idx] + pL + pR + pD + pU + pB + pF) * (1.0f / 6.0f);
     P[idx] = pc + omega * (projected - pc);
+}
+
+__global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const float* __restrict__ D, int parity, float omega, int nx, int ny, int nz) {
+    const int3 c = cell_of(xcd_block());
+    sor_half_at(P, D, parity, omega, nx, ny, nz, c.x, c.y, c.z);
 }
 
 // Temporally blocked red-black SOR. A workgroup owns a T^3 tile, loads p on
@@ -444,15 +462,13 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
 // d is read from global memory (L2) inside the sweep, which keeps the kernel
 // at ~31 VGPRs and 55 KB of LDS so that two workgroups share a CU.
 template <int T, int K>
-__global__ __launch_bounds__((T + 2 * K) * (T + 2 * K)) void k_sor_tb(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D,
-                                                                     float omega, int h0, int count, int nx, int ny, int nz) {
+__device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D, float omega, int h0,
+                                         int count, int nx, int ny, int nz, float* __restrict__ sp, uint3 blk) {
     constexpr int E = T + 2 * K;
     constexpr int NT = E * E;  // threads per workgroup
     constexpr int HX = E / 2;  // cells of one colour per LDS row
     static_assert(E % 2 == 0 && NT % 64 == 0, "tile edge plus halo must be even and E^2 a multiple of 64");
-    __shared__ float sp[E * E * E];
     const int tid = threadIdx.x;
-    const uint3 blk = xcd_block();
     const int ox = blk.x * T, oy = blk.y * T, oz = blk.z * T;
     const int bx = ox - K, by = oy - K, bz = oz - K;  // global coordinate of local 0
     const size_t row = static_cast<size_t>(nx), plane = row * ny;
@@ -519,13 +535,16 @@ __global__ __launch_bounds__((T + 2 * K) * (T + 2 * K)) void k_sor_tb(const floa
     }
 }
 
+template <int T, int K>
+__global__ __launch_bounds__((T + 2 * K) * (T + 2 * K)) void k_sor_tb(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D,
+                                                                     float omega, int h0, int count, int nx, int ny, int nz) {
+    __shared__ float sp[(T + 2 * K) * (T + 2 * K) * (T + 2 * K)];
+    sor_tile<T, K>(P, Pout, D, omega, h0, count, nx, ny, nz, sp, xcd_block());
+}
+
 // ----------------------------------------------------------- K13-K14 ----
-__global__ __launch_bounds__(256) void k_project(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, const float* __restrict__ P,
-                                                 DevParams p) {
-    const uint3 blk = xcd_block();
-    const int x = blk.x * blockDim.x + threadIdx.x;
-    const int y = blk.y * blockDim.y + threadIdx.y;
-    const int z = blk.z * blockDim.z + threadIdx.z;
+__device__ __forceinline__ void project_at(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, const float* __restrict__ P,
+                                           const DevParams& p, int x, int y, int z) {
     if (x > p.nx || y > p.ny || z > p.nz) return;
     if (y < p.ny && z < p.nz) {
         const size_t f = uidx(p, x, y, z);
@@ -542,6 +561,12 @@ __global__ __launch_bounds__(256) void k_project(float* __restrict__ U, float* _
         if (z == 0 || z == p.nz) W[f] = 0.0f;
         else W[f] -= P[sidx(p, x, y, z)] - P[sidx(p, x, y, z - 1)];
     }
+}
+
+__global__ __launch_bounds__(256) void k_project(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, const float* __restrict__ P,
+                                                 DevParams p) {
+    const int3 c = cell_of(xcd_block());
+    project_at(U, V, W, P, p, c.x, c.y, c.z);
 }
 
 // -------------------------------------------------------------- reset ----
@@ -694,6 +719,119 @@ __global__ void k_unpack_bf16(float* __restrict__ dst, const uint16_t* __restric
         dst[i] = __uint_as_float(static_cast<uint32_t>(src[i]) << 16);
 }
 
+// ============================================================ batches ====
+// Batched passes: one persistent launch covers every scene of a BatchTable.
+// Each walks the scenes' block lists and runs the same per-cell body as the
+// one-scene kernel above, on the block coordinates the one-scene launch would
+// give that tile, so results are identical to stepping each scene alone.
+template <bool B>
+__device__ __forceinline__ Scalars<B> scal_of(void* const* f) {
+    Scalars<B> s;
+#pragma unroll
+    for (int i = 0; i < NSCALAR; ++i) s.f[i] = f[i];
+    return s;
+}
+
+// block_sum_256 for use inside a loop: the trailing barrier keeps the next
+// iteration's writes from overtaking this iteration's read.
+__device__ __forceinline__ double block_sum_256_loop(double v) {
+    const double t = block_sum_256(v);
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(256) void kb_advance_clock(const BatchTable* __restrict__ T) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < T->count; s += gridDim.x * blockDim.x) {
+        float* c = T->scene[s].clock;
+        const float dt = T->scene[s].p.dt;
+        c[0] += dt;
+        c[1] += dt;
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_advect_velocity(const BatchTable* __restrict__ T) {
+    batch_blocks(T->faceStart, T->count, [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const int3 c = cell_of(block_of(local, S.faceG));
+        advect_velocity_at(S.velB[0], S.velB[1], S.velB[2], S.velA[0], S.velA[1], S.velA[2], S.p, c.x, c.y, c.z);
+    });
+}
+
+template <bool B>
+__global__ __launch_bounds__(256) void kb_advect_react(const BatchTable* __restrict__ T) {
+    batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const int3 c = cell_of(block_of(local, S.cellG));
+        float release = 0.0f;
+        if (c.x < S.p.nx && c.y < S.p.ny && c.z < S.p.nz)
+            release = react_cell<B>(scal_of<B>(S.sOut), scal_of<B>(S.sIn), S.velB[0], S.velB[1], S.velB[2], S.p, c.x, c.y, c.z);
+        const double t = block_sum_256_loop(static_cast<double>(release));
+        if (threadIdx.x == 0 && threadIdx.y == 0 && threadIdx.z == 0) S.partials[local] = t;
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_finalize_release(const BatchTable* __restrict__ T) {
+    for (int s = blockIdx.x; s < T->count; s += gridDim.x) {
+        const BatchScene& S = T->scene[s];
+        const size_t n = static_cast<size_t>(S.cellG[0]) * S.cellG[1] * S.cellG[2];
+        double sum = 0.0;
+        for (size_t i = threadIdx.x; i < n; i += blockDim.x) sum += S.partials[i];
+        const double t = block_sum_256_loop(sum);
+        if (threadIdx.x == 0) {
+            S.acc[0] = t;
+            S.acc[1] += t;
+        }
+    }
+}
+
+template <bool B>
+__global__ __launch_bounds__(256) void kb_force_contrib(const BatchTable* __restrict__ T) {
+    batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const int3 c = cell_of(block_of(local, S.cellG));
+        force_contrib_at<B>(S.cu, S.cv, S.cw, scal_of<B>(S.sOut), S.velB[0], S.velB[1], S.velB[2], S.p, c.x, c.y, c.z);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_force_apply(const BatchTable* __restrict__ T) {
+    batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const int3 c = cell_of(block_of(local, S.cellG));
+        force_apply_at(S.velA[0], S.velA[1], S.velA[2], S.velB[0], S.velB[1], S.velB[2], S.cu, S.cv, S.cw, S.d, S.p0, S.p, c.x, c.y, c.z);
+    });
+}
+
+// Pressure launch j: half-sweeps j*K .. j*K+K-1 of every scene that has them.
+template <int T_, int K>
+__global__ __launch_bounds__((T_ + 2 * K) * (T_ + 2 * K)) void kb_sor_tb(const BatchTable* __restrict__ T, int j) {
+    __shared__ float sp[(T_ + 2 * K) * (T_ + 2 * K) * (T_ + 2 * K)];
+    const int h0 = j * K;
+    batch_blocks(T->sorStart, T->sorActive[j], [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const float* in = (j & 1) ? S.p1 : S.p0;
+        float* out = (j & 1) ? S.p0 : S.p1;
+        sor_tile<T_, K>(in, out, S.d, S.p.omega, h0, min(K, S.halfSweeps - h0), S.p.nx, S.p.ny, S.p.nz, sp, block_of(local, S.sorG));
+        __syncthreads();  // the next tile reuses sp
+    });
+}
+
+// Pressure launch j of the plain kernel: half-sweep j (colour j & 1), in place.
+__global__ __launch_bounds__(256) void kb_sor_half(const BatchTable* __restrict__ T, int j) {
+    batch_blocks(T->sorStart, T->sorActive[j], [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const int3 c = cell_of(block_of(local, S.sorG));
+        sor_half_at(S.p0, S.d, j & 1, S.p.omega, S.p.nx, S.p.ny, S.p.nz, c.x, c.y, c.z);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_project(const BatchTable* __restrict__ T) {
+    batch_blocks(T->faceStart, T->count, [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        const int3 c = cell_of(block_of(local, S.faceG));
+        project_at(S.velA[0], S.velA[1], S.velA[2], S.pFinal, S.p, c.x, c.y, c.z);
+    });
+}
+
 // ========================================================== launchers ====
 template <bool B>
 static Scalars<B> scal(void* const* f) {
@@ -755,6 +893,19 @@ int sor_depth(int sorKernel) {
     }
 }
 
+// Tile edge of a temporally blocked SOR kernel (0 for the plain half-sweep kernel).
+int sor_tile_edge(int sorKernel) {
+    switch (sorKernel) {
+        case 1: return 20;
+        case 2: return 16;
+        case 3: return 8;
+        case 4: return 12;
+        case 5: return 8;
+        case 6: return 18;
+        default: return 0;
+    }
+}
+
 template <int T, int K>
 static void launch_sor_tb(DeviceScene& s, const DevParams& p, hipStream_t st) {
     const dim3 g((s.nx + T - 1) / T, (s.ny + T - 1) / T, (s.nz + T - 1) / T);
@@ -778,7 +929,7 @@ hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
         case 6: launch_sor_tb<18, 3>(s, p, st); break;
         default: {
             const dim3 b = cell_block();
-            const dim3 g(((s.nx + 1) / 2 + b.x - 1) / b.x, (s.ny + b.y - 1) / b.y, (s.nz + b.z - 1) / b.z);
+            const dim3 g = sor_grid(0, s.nx, s.ny, s.nz);
             for (int i = 0; i < p.iterations; ++i) {
                 k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 0, p.omega, s.nx, s.ny, s.nz);
                 k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 1, p.omega, s.nx, s.ny, s.nz);
@@ -786,6 +937,15 @@ hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
         }
     }
     return hipGetLastError();
+}
+
+dim3 sor_grid(int sorKernel, int nx, int ny, int nz) {
+    const int t = sor_tile_edge(sorKernel);
+    if (t == 0) {
+        const dim3 b = cell_block();
+        return dim3(((nx + 1) / 2 + b.x - 1) / b.x, (ny + b.y - 1) / b.y, (nz + b.z - 1) / b.z);
+    }
+    return dim3((nx + t - 1) / t, (ny + t - 1) / t, (nz + t - 1) / t);
 }
 
 int pressure_launch_count(const DeviceScene& s, int iterations) {
@@ -807,6 +967,56 @@ hipError_t launch_metrics(DeviceScene& s, const DevParams& p, hipStream_t st) {
         k_metrics<false><<<kMetricBlocks, 256, 0, st>>>(s.partials, scal<false>(s.s), s.u, s.v, s.w, s.p, s.d, s.light, s.shadow, p);
     k_reduce_metrics<<<1, 256, 0, st>>>(s.partials, kMetricBlocks, s.metricsOut);
     return hipGetLastError();
+}
+
+template <int T, int K>
+static void launch_batch_sor_tb(const BatchTable* table, int launches, int grid, hipStream_t st) {
+    constexpr int E = T + 2 * K;
+    // two E^3 LDS tiles fit a CU, so grid / 4 workgroups are resident at once
+    for (int j = 0; j < launches; ++j) kb_sor_tb<T, K><<<grid / 4, E * E, 0, st>>>(table, j);
+}
+
+hipError_t launch_batch_step(const BatchTable* table, bool bf16, int sorKernel, int sorLaunches, int grid, hipStream_t st) {
+    if (grid < 16 || grid % 16 != 0 || sorLaunches < 0 || sorLaunches > kMaxSorLaunches) return hipErrorInvalidValue;
+    const dim3 b = cell_block();
+    kb_advance_clock<<<2, 256, 0, st>>>(table);
+    kb_advect_velocity<<<grid, b, 0, st>>>(table);
+    if (bf16) kb_advect_react<true><<<grid, b, 0, st>>>(table);
+    else kb_advect_react<false><<<grid, b, 0, st>>>(table);
+    kb_finalize_release<<<kMaxBatch, 256, 0, st>>>(table);
+    if (bf16) kb_force_contrib<true><<<grid, b, 0, st>>>(table);
+    else kb_force_contrib<false><<<grid, b, 0, st>>>(table);
+    kb_force_apply<<<grid, b, 0, st>>>(table);
+    hipError_t e = launch_batch_pressure(table, sorKernel, sorLaunches, grid, st);
+    if (e != hipSuccess) return e;
+    kb_project<<<grid, b, 0, st>>>(table);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_batch_light(table, bf16, grid, st);
+}
+
+hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sorLaunches, int grid, hipStream_t st) {
+    if (grid < 16 || grid % 16 != 0 || sorLaunches < 0 || sorLaunches > kMaxSorLaunches) return hipErrorInvalidValue;
+    switch (sorKernel) {
+        case 1: launch_batch_sor_tb<20, 2>(table, sorLaunches, grid, st); break;
+        case 2: launch_batch_sor_tb<16, 4>(table, sorLaunches, grid, st); break;
+        case 3: launch_batch_sor_tb<8, 4>(table, sorLaunches, grid, st); break;
+        case 4: launch_batch_sor_tb<12, 6>(table, sorLaunches, grid, st); break;
+        case 5: launch_batch_sor_tb<8, 8>(table, sorLaunches, grid, st); break;
+        case 6: launch_batch_sor_tb<18, 3>(table, sorLaunches, grid, st); break;
+        default:
+            for (int j = 0; j < sorLaunches; ++j) kb_sor_half<<<grid, cell_block(), 0, st>>>(table, j);
+    }
+    return hipGetLastError();
+}
+
+// Eight 256-thread workgroups per CU: enough resident waves to hide memory
+// latency in the cell passes (the SOR tile kernels use half as many).
+int batch_grid(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 2048;
+    const int g = prop.multiProcessorCount * 8;
+    return g < 16 ? 16 : g - g % 16;
 }
 
 hipError_t launch_pack_scalar(const DeviceScene& s, int field, const float* src, hipStream_t st) {

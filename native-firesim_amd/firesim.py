@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, Structure, byref, c_char_p, c_float, c_int32, c_uint32, c_uint64, c_void_p, c_double
+from ctypes import POINTER, Structure, byref, c_char_p, c_double, c_float, c_int32, c_int64, c_uint32, c_uint64, c_void_p
 
 import numpy as np
 
@@ -94,6 +94,39 @@ class Metrics(Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class ServerDesc(Structure):
+    _fields_ = [("device", c_int32), ("max_active", c_int32), ("hbm_budget_bytes", c_uint64),
+                ("sor_kernel", c_int32), ("use_graph", c_int32)]
+
+
+class Request(Structure):
+    _fields_ = [("desc", SceneDesc), ("params", Params), ("emitters", Emitter * MAX_EMITTERS), ("emitter_count", c_int32),
+                ("steps", c_int32), ("render_every", c_int32), ("keep_scene", c_int32), ("arrival_s", c_double)]
+
+
+class RequestStatus(Structure):
+    _fields_ = [("state", c_int32), ("steps_done", c_int32), ("frames", c_int32), ("reserved", c_int32),
+                ("arrival_s", c_double), ("admitted_s", c_double), ("finished_s", c_double),
+                ("step_ms_p50", c_double), ("step_ms_p99", c_double), ("step_ms_max", c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class ServerStats(Structure):
+    _fields_ = [("requests_submitted", c_int64), ("requests_done", c_int64), ("requests_failed", c_int64),
+                ("steps", c_int64), ("voxel_steps", c_int64), ("rounds", c_int64), ("frames", c_int64),
+                ("max_active_seen", c_int32), ("sor_kernel", c_int32), ("peak_resident_bytes", c_uint64),
+                ("busy_s", c_double), ("step_ms_p50", c_double), ("step_ms_p99", c_double), ("step_ms_mean", c_double),
+                ("step_ms_max", c_double), ("queue_ms_p50", c_double), ("queue_ms_p99", c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+REQ_PENDING, REQ_WAITING, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(5)
 
 
 class Tuning(Structure):
@@ -220,6 +253,33 @@ def load_library(path: str = LIB_PATH):
         "fs_write_field": (c_int32, [c_void_p, c_int32, P(c_float)]),
         "fs_metrics_collect": (c_int32, [c_void_p, P(Metrics)]),
         "fs_stream": (c_void_p, [c_void_p]),
+        "fs_batch_create": (c_int32, [c_int32, P(c_void_p)]),
+        "fs_batch_destroy": (c_int32, [c_void_p]),
+        "fs_batch_add": (c_int32, [c_void_p, c_void_p]),
+        "fs_batch_remove": (c_int32, [c_void_p, c_void_p]),
+        "fs_batch_count": (c_int32, [c_void_p, P(c_int32)]),
+        "fs_batch_set_tuning": (c_int32, [c_void_p, P(Tuning)]),
+        "fs_batch_get_tuning": (c_int32, [c_void_p, P(Tuning)]),
+        "fs_batch_autotune": (c_int32, [c_void_p, c_int32, P(AutotuneReport)]),
+        "fs_batch_step": (c_int32, [c_void_p, c_int32]),
+        "fs_batch_step_latency": (c_int32, [c_void_p, c_int32, P(c_double)]),
+        "fs_batch_synchronize": (c_int32, [c_void_p]),
+        "fs_batch_last_error": (c_char_p, [c_void_p]),
+        "fs_batch_stream": (c_void_p, [c_void_p]),
+        "fs_create_in": (c_int32, [P(SceneDesc), c_void_p, c_uint64, P(c_void_p)]),
+        "fs_render_bgra8_async": (c_int32, [c_void_p, P(c_uint32)]),
+        "fs_server_create": (c_int32, [P(ServerDesc), P(c_void_p)]),
+        "fs_server_destroy": (c_int32, [c_void_p]),
+        "fs_server_submit": (c_int32, [c_void_p, P(Request), P(c_uint64)]),
+        "fs_server_tick": (c_int32, [c_void_p, P(c_int32)]),
+        "fs_server_run": (c_int32, [c_void_p]),
+        "fs_server_status": (c_int32, [c_void_p, c_uint64, P(RequestStatus)]),
+        "fs_server_frame": (c_int32, [c_void_p, c_uint64, P(c_uint32), P(c_int32)]),
+        "fs_server_take_scene": (c_int32, [c_void_p, c_uint64, P(c_void_p)]),
+        "fs_server_stats_get": (c_int32, [c_void_p, P(ServerStats)]),
+        "fs_server_step_latencies": (c_int32, [c_void_p, P(c_double), c_int64, P(c_int64)]),
+        "fs_server_clock": (c_double, [c_void_p]),
+        "fs_server_last_error": (c_char_p, [c_void_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -413,3 +473,180 @@ def write_bmp(path: str, bgra: np.ndarray) -> None:
     st = lib.fs_write_bmp(path.encode(), arr.ctypes.data_as(POINTER(c_uint32)), w, h)
     if st != FS_OK:
         raise FireSimError(f"fs_write_bmp failed ({st}): {lib.fs_last_global_error().decode()}")
+
+
+class Batch:
+    """Many scenes stepped together, one launch per pass (fs_batch)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = c_void_p()
+        st = self.lib.fs_batch_create(device, byref(h))
+        if st != FS_OK:
+            raise FireSimError(f"fs_batch_create failed ({st}): {self.lib.fs_last_global_error().decode()}")
+        self.h = h
+        self.members: list = []
+
+    def _check(self, st: int, what: str):
+        if st != FS_OK:
+            msg = self.lib.fs_batch_last_error(self.h)
+            raise FireSimError(f"{what} failed ({st}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fs_batch_destroy(self.h)
+            self.h = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, scene: Scene):
+        self._check(self.lib.fs_batch_add(self.h, scene.h), "fs_batch_add")
+        self.members.append(scene)
+
+    def remove(self, scene: Scene):
+        self._check(self.lib.fs_batch_remove(self.h, scene.h), "fs_batch_remove")
+        self.members.remove(scene)
+
+    def count(self) -> int:
+        n = c_int32()
+        self._check(self.lib.fs_batch_count(self.h, byref(n)), "fs_batch_count")
+        return n.value
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        self._check(self.lib.fs_batch_get_tuning(self.h, byref(t)), "fs_batch_get_tuning")
+        return t
+
+    def set_tuning(self, sor_kernel=None, use_graph=None):
+        t = self.tuning()
+        if sor_kernel is not None:
+            t.sor_kernel = SOR_KERNELS.index(sor_kernel) if isinstance(sor_kernel, str) else sor_kernel
+        if use_graph is not None:
+            t.use_graph = int(bool(use_graph))
+        self._check(self.lib.fs_batch_set_tuning(self.h, byref(t)), "fs_batch_set_tuning")
+
+    def autotune(self, reps: int = 2) -> dict:
+        r = AutotuneReport()
+        self._check(self.lib.fs_batch_autotune(self.h, reps, byref(r)), "fs_batch_autotune")
+        return {"sor_ms": {SOR_KERNELS[i]: r.sor_ms[i] for i in range(len(SOR_KERNELS))}, "chosen": SOR_KERNELS[r.chosen]}
+
+    def step(self, n: int = 1):
+        self._check(self.lib.fs_batch_step(self.h, n), "fs_batch_step")
+
+    def step_latency(self, n: int) -> np.ndarray:
+        out = np.zeros(max(1, n), dtype=np.float64)
+        self._check(self.lib.fs_batch_step_latency(self.h, n, out.ctypes.data_as(POINTER(c_double))), "fs_batch_step_latency")
+        return out[:n]
+
+    def synchronize(self):
+        self._check(self.lib.fs_batch_synchronize(self.h), "fs_batch_synchronize")
+
+    def stream(self) -> int:
+        return self.lib.fs_batch_stream(self.h)
+
+
+def make_request(nx, ny, nz, params: Params, emitters=(), steps=60, render_every=0, frame=(256, 256),
+                 storage=STORAGE_FP32, keep_scene=False, arrival_s=-1.0) -> Request:
+    """An fs_request for Server.submit."""
+    r = Request()
+    r.desc = SceneDesc(nx, ny, nz, frame[0], frame[1], storage, 0)
+    r.params = params
+    for i, e in enumerate(emitters):
+        r.emitters[i] = e if isinstance(e, Emitter) else Emitter(*e)
+    r.emitter_count = len(emitters)
+    r.steps, r.render_every, r.keep_scene, r.arrival_s = steps, render_every, int(bool(keep_scene)), arrival_s
+    return r
+
+
+class Server:
+    """Continuous batching of simulation requests (fs_server)."""
+
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 256, sor_kernel: int = -1, use_graph: bool = True,
+                 device: int = 0):
+        self.lib = load_library()
+        d = ServerDesc(device, max_active, hbm_budget_bytes, sor_kernel, int(use_graph))
+        h = c_void_p()
+        st = self.lib.fs_server_create(byref(d), byref(h))
+        if st != FS_OK:
+            raise FireSimError(f"fs_server_create failed ({st}): {self.lib.fs_last_global_error().decode()}")
+        self.h = h
+
+    def _check(self, st: int, what: str):
+        if st != FS_OK:
+            msg = self.lib.fs_server_last_error(self.h)
+            raise FireSimError(f"{what} failed ({st}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fs_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, req: Request) -> int:
+        rid = c_uint64()
+        self._check(self.lib.fs_server_submit(self.h, byref(req), byref(rid)), "fs_server_submit")
+        return rid.value
+
+    def tick(self) -> int:
+        n = c_int32()
+        self._check(self.lib.fs_server_tick(self.h, byref(n)), "fs_server_tick")
+        return n.value
+
+    def run(self):
+        self._check(self.lib.fs_server_run(self.h), "fs_server_run")
+
+    def status(self, rid: int) -> dict:
+        s = RequestStatus()
+        self._check(self.lib.fs_server_status(self.h, rid, byref(s)), "fs_server_status")
+        return s.as_dict()
+
+    def frame(self, rid: int, w: int, h: int):
+        out = np.empty((h, w), dtype=np.uint32)
+        step = c_int32()
+        self._check(self.lib.fs_server_frame(self.h, rid, out.ctypes.data_as(POINTER(c_uint32)), byref(step)), "fs_server_frame")
+        return out, step.value
+
+    def take_scene(self, rid: int) -> Scene:
+        h = c_void_p()
+        self._check(self.lib.fs_server_take_scene(self.h, rid, byref(h)), "fs_server_take_scene")
+        sc = Scene.attach(h.value)
+        sc._owned = True  # the caller now owns the scene
+        return sc
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        self._check(self.lib.fs_server_stats_get(self.h, byref(s)), "fs_server_stats_get")
+        return s.as_dict()
+
+    def latencies(self) -> np.ndarray:
+        n = c_int64()
+        self._check(self.lib.fs_server_step_latencies(self.h, None, 0, byref(n)), "fs_server_step_latencies")
+        out = np.zeros(max(1, n.value), dtype=np.float64)
+        self._check(self.lib.fs_server_step_latencies(self.h, out.ctypes.data_as(POINTER(c_double)), n.value, byref(n)),
+                    "fs_server_step_latencies")
+        return out[:n.value]
+
+    def clock(self) -> float:
+        return self.lib.fs_server_clock(self.h)
