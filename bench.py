@@ -17,7 +17,15 @@ Printed: one JSON line with the contract keys plus
   cpu_baseline  the FP64 oracle (OpenMP) on this host, bounded sample,
   stages        per-pass times and algorithmic-bandwidth fractions,
   step_latency  p50/p99 of the per-step GPU times in the timed region,
-  render        one HDR raymarch + tonemap frame after the timed steps.
+  render        one HDR raymarch + tonemap frame after the timed steps,
+  served        a Poisson stream of requests (cubic grids 32..192, lognormal
+                step counts, BF16 heat/fuel/soot, a 256x256 frame every 10
+                steps) through the continuous-batching server: p50/p99 step
+                latency and aggregate voxel-steps/s, against the same requests
+                run one at a time on the plain FP32 path,
+  batched       48 uneven scenes (48^3..128^3, 64x128x64, SOR 30-100) in one
+                persistent launch per pass, FP32 and BF16, against the same
+                scenes one at a time on the plain FP32 path.
 """
 from __future__ import annotations
 
@@ -34,6 +42,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "native-firesim_amd"))
 
 import firesim  # noqa: E402
+import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 
@@ -53,37 +62,11 @@ PASS_KERNELS = {
     "light": ["k_light_build", "k_light_propagate"],
 }
 
-CONFIGS = {
-    "c1": dict(grid=(64, 64, 64), sor=40, turbulence=0.0, wind=False, emitters="single", render=(256, 256, 104),
-               desc="64^3 FP32, emitter r=6 at (32,6,32), SOR 40 x omega 1.7"),
-    "c2": dict(grid=(128, 128, 128), sor=80, turbulence=0.5, wind=True, emitters="single", render=(512, 512, 128),
-               desc="128^3 FP32, emitter r=12 at (64,12,64), seeded wind 0-2 voxel/s, turbulence 0.5, SOR 80 x omega 1.7"),
-    "c3": dict(grid=(256, 384, 256), sor=120, turbulence=0.5, wind=True, emitters="eight", render=(1280, 720, 104),
-               desc="256x384x256 FP32, 8 seeded floor emitters, SOR 120 x omega 1.7"),
-}
+CONFIGS = workloads.SINGLE
 
 
 def workload(name: str, seed: int):
-    cfg = CONFIGS[name]
-    nx, ny, nz = cfg["grid"]
-    rng = np.random.default_rng(seed)
-    wind_x = wind_z = 0.0
-    if cfg["wind"]:
-        speed, theta = rng.uniform(0.0, 2.0), rng.uniform(0.0, 2.0 * math.pi)
-        # voxels/s -> domain fractions/s per axis (the step's velocity unit)
-        wind_x, wind_z = speed * math.cos(theta) / nx, speed * math.sin(theta) / nz
-    if cfg["emitters"] == "single":
-        r = 6.0 * nx / 64.0
-        emitters = [(nx / 2, r, nz / 2, r, 1.0, 1500.0)]
-    else:
-        emitters = []
-        for _ in range(8):
-            r = rng.uniform(4.0, 10.0)
-            emitters.append((rng.uniform(r, nx - r), r, rng.uniform(r, nz - r), r, rng.uniform(0.5, 1.5), rng.uniform(1200.0, 1800.0)))
-    params = firesim.default_params(scene_id=firesim.SCENE_EMITTERS_ONLY, turbulence=cfg["turbulence"], wind_x=wind_x, wind_z=wind_z,
-                                    sor_omega=1.7, sor_iterations=cfg["sor"], raymarch_steps=cfg["render"][2], noise_seed=seed,
-                                    camera_distance=3.0, cinematic_mode=1)
-    return cfg, params, emitters
+    return workloads.single_scene(name, seed)
 
 
 def cpu_baseline(name: str, seed: int, budget_s: float):
@@ -141,6 +124,113 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
 
 
+def make_scene(grid, params, emitters, storage=firesim.STORAGE_FP32, frame=(64, 64)):
+    s = firesim.Scene(*grid, frame[0], frame[1], storage)
+    s.set_params(params)
+    s.set_emitters(emitters)
+    s.reset()
+    return s
+
+
+def pcts(ms) -> dict:
+    ms = np.asarray(ms, dtype=np.float64)
+    return {"p50": round(float(np.percentile(ms, 50)), 4), "p99": round(float(np.percentile(ms, 99)), 4),
+            "mean": round(float(np.mean(ms)), 4)} if ms.size else None
+
+
+def plain_serial_step_ms(grid, params, emitters, steps):
+    """The plain one-scene-at-a-time FP32 path: one scene, reference launch
+    structure (one launch per SOR half-sweep, direct launches, no graph)."""
+    s = make_scene(grid, params, emitters)
+    s.set_tuning(sor_kernel=0, use_graph=False)
+    s.step(1)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    s.close()
+    return dt * 1e3
+
+
+def bench_batched(seed: int, steps: int, count: int):
+    """Uneven batch: `count` scenes of the batch shapes in one persistent
+    launch per pass, FP32 and BF16 storage, against the same scenes stepped
+    one at a time on the plain FP32 path. (Per-field error of batched FP32
+    and BF16 scenes against the FP64 oracle is a GPU test.)"""
+    specs = workloads.batch_scenes(seed, count)
+    cells = sum(math.prod(sp["grid"]) for sp in specs)
+    result = {"scenes": count, "shapes": [list(s) for s in workloads.BATCH_SHAPES], "cells_per_step": cells,
+              "sor_iterations": [int(sp["params"].sor_iterations) for sp in specs]}
+    for mode, storage in (("fp32", firesim.STORAGE_FP32), ("bf16", firesim.STORAGE_BF16)):
+        scenes = [make_scene(sp["grid"], sp["params"], sp["emitters"], storage) for sp in specs]
+        with firesim.Batch() as batch:
+            for sc in scenes:
+                batch.add(sc)
+            batch.step(2)
+            tuned = batch.autotune(reps=1)
+            batch.step(1)
+            batch.synchronize()
+            t0 = time.perf_counter()
+            ms = batch.step_latency(steps)
+            wall = time.perf_counter() - t0
+        for sc in scenes:
+            sc.close()
+        result[mode] = {"voxel_steps_per_s": cells * steps / wall, "ms_per_step": wall * 1e3 / steps, "step_ms": pcts(ms),
+                        "sor_kernel": tuned["chosen"]}
+    # the same scenes one at a time on the plain FP32 path (3 timed steps each)
+    plain_ms = sum(plain_serial_step_ms(sp["grid"], sp["params"], sp["emitters"], 3) for sp in specs)
+    result["plain_fp32_serial"] = {"voxel_steps_per_s": cells / (plain_ms * 1e-3), "ms_per_step": plain_ms}
+    result["speedup_vs_plain_fp32"] = {m: result[m]["voxel_steps_per_s"] / result["plain_fp32_serial"]["voxel_steps_per_s"] for m in ("fp32", "bf16")}
+    return result
+
+
+def bench_served(seed: int, count: int, rate: float, budget_gb: float, storage: int):
+    """Served stream: Poisson requests through fs_server (continuous batching,
+    HBM-budget admission, a frame every 10 steps), then the same requests
+    serially through the plain FP32 path."""
+    reqs = workloads.served_requests(seed, count, rate)
+    total_voxel_steps = sum(math.prod(r["grid"]) * r["steps"] for r in reqs)
+    with firesim.Server(hbm_budget_bytes=int(budget_gb * (1 << 30)), max_active=256, sor_kernel=-1) as sv:
+        for r in reqs:
+            sv.submit(firesim.make_request(*r["grid"], r["params"], r["emitters"], steps=r["steps"], render_every=r["render_every"],
+                                           frame=r["frame"], storage=storage, arrival_s=r["arrival"]))
+        t0 = time.perf_counter()
+        sv.run()
+        wall = time.perf_counter() - t0
+        st = sv.stats()
+    served = {"requests": count, "arrival_rate_per_s": rate, "hbm_budget_gb": budget_gb,
+              "storage": "bf16" if storage == firesim.STORAGE_BF16 else "fp32",
+              "voxel_steps": st["voxel_steps"], "busy_s": round(st["busy_s"], 3), "wall_s": round(wall, 3),
+              "voxel_steps_per_s": st["voxel_steps"] / st["busy_s"] if st["busy_s"] > 0 else None,
+              "step_ms": {"p50": round(st["step_ms_p50"], 3), "p99": round(st["step_ms_p99"], 3), "mean": round(st["step_ms_mean"], 3),
+                          "max": round(st["step_ms_max"], 3)},
+              "queue_ms": {"p50": round(st["queue_ms_p50"], 3), "p99": round(st["queue_ms_p99"], 3)},
+              "max_active": st["max_active_seen"], "rounds": st["rounds"], "frames": st["frames"],
+              "peak_resident_gb": round(st["peak_resident_bytes"] / (1 << 30), 2), "sor_kernel": firesim.SOR_KERNELS[st["sor_kernel"]],
+              "failed": st["requests_failed"]}
+    assert st["voxel_steps"] == total_voxel_steps and st["requests_failed"] == 0
+    # serially, plain FP32 path, same requests back to back (arrivals ignored: the GPU never waits)
+    lat = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        s = make_scene(r["grid"], r["params"], r["emitters"], firesim.STORAGE_FP32, r["frame"])
+        s.set_tuning(sor_kernel=0, use_graph=False)
+        for k in range(1, r["steps"] + 1):
+            a = time.perf_counter()
+            s.step(1)
+            if k % r["render_every"] == 0 or k == r["steps"]:
+                s.render_bgra8()
+            else:
+                s.synchronize()
+            lat.append((time.perf_counter() - a) * 1e3)
+        s.close()
+    serial_wall = time.perf_counter() - t0
+    served["plain_fp32_serial"] = {"voxel_steps_per_s": total_voxel_steps / serial_wall, "wall_s": round(serial_wall, 3), "step_ms": pcts(lat)}
+    served["speedup_vs_plain_fp32"] = served["voxel_steps_per_s"] / served["plain_fp32_serial"]["voxel_steps_per_s"]
+    return served
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +244,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="run only the 256 MiB copy used to scale PMC counters")
+    ap.add_argument("--no-served", action="store_true", help="skip the served request stream")
+    ap.add_argument("--served-requests", type=int, default=80)
+    ap.add_argument("--served-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--served-budget-gb", type=float, default=200.0)
+    ap.add_argument("--served-fp32", action="store_true", help="serve FP32 fields instead of BF16 heat/fuel/soot")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 48-scene uneven batch")
+    ap.add_argument("--batched-scenes", type=int, default=48)
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--only", choices=["single", "batched", "served"], default=None, help="run one section only")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -163,6 +262,15 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
+    if args.only == "batched":
+        print(json.dumps({"batched": bench_batched(args.seed, args.batched_steps, args.batched_scenes)}))
+        return
+    if args.only == "served":
+        storage = firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16
+        print(json.dumps({"served": bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb, storage)}))
+        return
+    if args.only == "single":
+        args.no_batched = args.no_served = True
 
     cfg, params, emitters = workload(args.config, args.seed)
     nx, ny, nz = cfg["grid"]
@@ -223,6 +331,9 @@ def main():
     scene.close()
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.config, args.seed, args.cpu_budget)
+    batched = None if args.no_batched else bench_batched(args.seed, args.batched_steps, args.batched_scenes)
+    served = None if args.no_served else bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb,
+                                                      firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16)
     out = {
         "metric": "voxel-steps/s at 128³ + served p50/p99 step latency vs plain FP32 path",
         "value": value,
@@ -250,6 +361,8 @@ def main():
         "render": render,
         "sanity": {"invalid_cells": metrics["invalid_cells"], "heat_release_total": metrics["heat_release_total"],
                    "divergence_after_l2": metrics["divergence_after_l2"]},
+        "served": served,
+        "batched": batched,
     }
     print(json.dumps(out))
 

@@ -193,21 +193,30 @@ struct Tri {
     float tx, ty, tz;
 };
 
-__device__ __forceinline__ Tri tri_setup(int dx, int dy, int dz, float sx, float sy, float sz, float u, float v, float w) {
-    u = sat(u);
-    v = sat(v);
-    w = sat(w);
-    const float fx = u * sx, fy = v * sy, fz = w * sz;
-    const int x0 = static_cast<int>(floorf(fx));
-    const int y0 = static_cast<int>(floorf(fy));
-    const int z0 = static_cast<int>(floorf(fz));
+// Coordinates may be float or double. The step's backtraces use double: the
+// sample position f = u * (n - 1) reaches ~n cells, where one float ulp is up
+// to 1.5e-5 of a cell, and that position error times the field gradient would
+// dominate the FP32 error of the sampled value. The blend weights are float.
+__device__ __forceinline__ float sat_c(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+__device__ __forceinline__ double sat_c(double x) { return fmin(fmax(x, 0.0), 1.0); }
+
+template <class T>
+__device__ __forceinline__ Tri tri_setup(int dx, int dy, int dz, T sx, T sy, T sz, T u, T v, T w) {
+    u = sat_c(u);
+    v = sat_c(v);
+    w = sat_c(w);
+    const T fx = u * sx, fy = v * sy, fz = w * sz;
+    const T flx = floor(fx), fly = floor(fy), flz = floor(fz);
+    const int x0 = static_cast<int>(flx);
+    const int y0 = static_cast<int>(fly);
+    const int z0 = static_cast<int>(flz);
     const int x1 = min(x0 + 1, dx - 1);
     const int y1 = min(y0 + 1, dy - 1);
     const int z1 = min(z0 + 1, dz - 1);
     Tri t;
-    t.tx = fx - static_cast<float>(x0);
-    t.ty = fy - static_cast<float>(y0);
-    t.tz = fz - static_cast<float>(z0);
+    t.tx = static_cast<float>(fx - flx);
+    t.ty = static_cast<float>(fy - fly);
+    t.tz = static_cast<float>(fz - flz);
     const size_t r00 = (static_cast<size_t>(z0) * dy + y0) * dx;
     const size_t r10 = (static_cast<size_t>(z0) * dy + y1) * dx;
     const size_t r01 = (static_cast<size_t>(z1) * dy + y0) * dx;
@@ -238,31 +247,32 @@ __device__ __forceinline__ float tri_scalar(const Scalars<B>& S, const Tri& t) {
                      S.template ld<K>(t.i001), S.template ld<K>(t.i101), S.template ld<K>(t.i011), S.template ld<K>(t.i111));
 }
 
-__device__ __forceinline__ Tri tri_scalar_setup(const DevParams& p, float u, float v, float w) {
-    return tri_setup(p.nx, p.ny, p.nz, static_cast<float>(p.nx - 1), static_cast<float>(p.ny - 1), static_cast<float>(p.nz - 1), u, v,
-                     w);
+template <class T>
+__device__ __forceinline__ Tri tri_scalar_setup(const DevParams& p, T u, T v, T w) {
+    return tri_setup<T>(p.nx, p.ny, p.nz, static_cast<T>(p.nx - 1), static_cast<T>(p.ny - 1), static_cast<T>(p.nz - 1), u, v, w);
 }
 
 template <int K, bool B>
 __device__ __forceinline__ float sample_scalar(const Scalars<B>& S, const DevParams& p, float u, float v, float w) {
-    return tri_scalar<K, B>(S, tri_scalar_setup(p, u, v, w));
+    return tri_scalar<K, B>(S, tri_scalar_setup<float>(p, u, v, w));
 }
 
-__device__ __forceinline__ float sample_u(const float* __restrict__ f, const DevParams& p, float u, float v, float w) {
-    return tri_plain(f, tri_setup(p.nx + 1, p.ny, p.nz, static_cast<float>(p.nx), static_cast<float>(p.ny - 1),
-                                  static_cast<float>(p.nz - 1), u, v, w));
+template <class T>
+__device__ __forceinline__ float sample_u(const float* __restrict__ f, const DevParams& p, T u, T v, T w) {
+    return tri_plain(f, tri_setup<T>(p.nx + 1, p.ny, p.nz, static_cast<T>(p.nx), static_cast<T>(p.ny - 1), static_cast<T>(p.nz - 1), u, v, w));
 }
-__device__ __forceinline__ float sample_v(const float* __restrict__ f, const DevParams& p, float u, float v, float w) {
-    return tri_plain(f, tri_setup(p.nx, p.ny + 1, p.nz, static_cast<float>(p.nx - 1), static_cast<float>(p.ny),
-                                  static_cast<float>(p.nz - 1), u, v, w));
+template <class T>
+__device__ __forceinline__ float sample_v(const float* __restrict__ f, const DevParams& p, T u, T v, T w) {
+    return tri_plain(f, tri_setup<T>(p.nx, p.ny + 1, p.nz, static_cast<T>(p.nx - 1), static_cast<T>(p.ny), static_cast<T>(p.nz - 1), u, v, w));
 }
-__device__ __forceinline__ float sample_w(const float* __restrict__ f, const DevParams& p, float u, float v, float w) {
-    return tri_plain(f, tri_setup(p.nx, p.ny, p.nz + 1, static_cast<float>(p.nx - 1), static_cast<float>(p.ny - 1),
-                                  static_cast<float>(p.nz), u, v, w));
+template <class T>
+__device__ __forceinline__ float sample_w(const float* __restrict__ f, const DevParams& p, T u, T v, T w) {
+    return tri_plain(f, tri_setup<T>(p.nx, p.ny, p.nz + 1, static_cast<T>(p.nx - 1), static_cast<T>(p.ny - 1), static_cast<T>(p.nz), u, v, w));
 }
+template <class T>
 __device__ __forceinline__ F3 sample_vel(const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                                         const DevParams& p, float u, float v, float w) {
-    return f3(sample_u(U, p, u, v, w), sample_v(V, p, u, v, w), sample_w(W, p, u, v, w));
+                                         const DevParams& p, T u, T v, T w) {
+    return f3(sample_u<T>(U, p, u, v, w), sample_v<T>(V, p, u, v, w), sample_w<T>(W, p, u, v, w));
 }
 
 // Light grid: float4 trilinear and nearest shadow (src/fire_cuda.cu:618-664).

@@ -54,34 +54,33 @@ __device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float
                                                    const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                    const DevParams& p, int x, int y, int z) {
     if (x > p.nx || y > p.ny || z > p.nz) return;
-    const float fnx = static_cast<float>(p.nx), fny = static_cast<float>(p.ny), fnz = static_cast<float>(p.nz);
+    // backtrace positions in double (see tri_setup); sampled values and outputs in float
+    const double dnx = p.nx, dny = p.ny, dnz = p.nz;
+    const double k = static_cast<double>(p.dt) * static_cast<double>(kAdvectionScale);
     if (y < p.ny && z < p.nz) {
         float out = 0.0f;
         if (x != 0 && x != p.nx) {
-            const float fu = static_cast<float>(x) / fnx, fv = (static_cast<float>(y) + 0.5f) / fny, fw = (static_cast<float>(z) + 0.5f) / fnz;
-            const F3 vel = sample_vel(U, V, W, p, fu, fv, fw);
-            out = sample_u(U, p, fu - vel.x * p.dt * kAdvectionScale, fv - vel.y * p.dt * kAdvectionScale, fw - vel.z * p.dt * kAdvectionScale) *
-                  0.995f;
+            const double fu = x / dnx, fv = (y + 0.5) / dny, fw = (z + 0.5) / dnz;
+            const F3 vel = sample_vel<double>(U, V, W, p, fu, fv, fw);
+            out = sample_u<double>(U, p, fu - vel.x * k, fv - vel.y * k, fw - vel.z * k) * 0.995f;
         }
         uo[uidx(p, x, y, z)] = out;
     }
     if (x < p.nx && z < p.nz) {
         float out = 0.0f;
         if (y != 0) {
-            const float fu = (static_cast<float>(x) + 0.5f) / fnx, fv = static_cast<float>(y) / fny, fw = (static_cast<float>(z) + 0.5f) / fnz;
-            const F3 vel = sample_vel(U, V, W, p, fu, fv, fw);
-            out = sample_v(V, p, fu - vel.x * p.dt * kAdvectionScale, fv - vel.y * p.dt * kAdvectionScale, fw - vel.z * p.dt * kAdvectionScale) *
-                  0.996f;
+            const double fu = (x + 0.5) / dnx, fv = y / dny, fw = (z + 0.5) / dnz;
+            const F3 vel = sample_vel<double>(U, V, W, p, fu, fv, fw);
+            out = sample_v<double>(V, p, fu - vel.x * k, fv - vel.y * k, fw - vel.z * k) * 0.996f;
         }
         vo[vidx(p, x, y, z)] = out;
     }
     if (x < p.nx && y < p.ny) {
         float out = 0.0f;
         if (z != 0 && z != p.nz) {
-            const float fu = (static_cast<float>(x) + 0.5f) / fnx, fv = (static_cast<float>(y) + 0.5f) / fny, fw = static_cast<float>(z) / fnz;
-            const F3 vel = sample_vel(U, V, W, p, fu, fv, fw);
-            out = sample_w(W, p, fu - vel.x * p.dt * kAdvectionScale, fv - vel.y * p.dt * kAdvectionScale, fw - vel.z * p.dt * kAdvectionScale) *
-                  0.995f;
+            const double fu = (x + 0.5) / dnx, fv = (y + 0.5) / dny, fw = z / dnz;
+            const F3 vel = sample_vel<double>(U, V, W, p, fu, fv, fw);
+            out = sample_w<double>(W, p, fu - vel.x * k, fv - vel.y * k, fw - vel.z * k) * 0.995f;
         }
         wo[widx(p, x, y, z)] = out;
     }
@@ -152,11 +151,14 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
         out.template st<SOPT>(idx, 0.0f);
         return 0.0f;
     }
-    const F3 vel = sample_vel(U, V, W, p, u, v, w);
-    const float pu = u - vel.x * dt * kAdvectionScale, pv = v - vel.y * dt * kAdvectionScale, pw = w - vel.z * dt * kAdvectionScale;
-    const F3 pvel = sample_vel(U, V, W, p, pu, pv, pw);
-    const Tri tp = tri_scalar_setup(p, pu, pv, pw);
-    const Tri tf = tri_scalar_setup(p, pu + pvel.x * dt * kAdvectionScale, pv + pvel.y * dt * kAdvectionScale, pw + pvel.z * dt * kAdvectionScale);
+    // backtrace and forward-trace positions in double (see tri_setup)
+    const double du = (x + 0.5) / static_cast<double>(p.nx), dv = (y + 0.5) / static_cast<double>(p.ny), dw = (z + 0.5) / static_cast<double>(p.nz);
+    const double k = static_cast<double>(dt) * static_cast<double>(kAdvectionScale);
+    const F3 vel = sample_vel<double>(U, V, W, p, du, dv, dw);
+    const double pu = du - vel.x * k, pv = dv - vel.y * k, pw = dw - vel.z * k;
+    const F3 pvel = sample_vel<double>(U, V, W, p, pu, pv, pw);
+    const Tri tp = tri_scalar_setup<double>(p, pu, pv, pw);
+    const Tri tf = tri_scalar_setup<double>(p, pu + pvel.x * k, pv + pvel.y * k, pw + pvel.z * k);
     const float anti = sat(0.30f + p.detail * 0.34f);
     float heat = maccormack<HEAT, B>(in, tp, tf, h0, anti);
     float fuel = maccormack<FUEL, B>(in, tp, tf, f0, anti * 0.92f);
@@ -364,26 +366,29 @@ __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, f
 
 // New face values after the gathered force and the boundary clamp
 // (velocityBoundaryKernel :1261-1291). Both cells sharing a face evaluate the
-// same expression, so the face is bit-identical from either side.
-__device__ __forceinline__ float face_u(const float* U, const float* cu, const DevParams& p, int fx, int y, int z) {
-    if (fx == 0 || fx == p.nx) return 0.0f;
-    float val = U[uidx(p, fx, y, z)];
+// same expression, so the face is bit-identical from either side. The sum is
+// formed in FP64 and rounded once when stored; the divergence below uses the
+// unrounded values, since it is a difference of nearly equal face values whose
+// FP32 rounding would dominate its relative error.
+__device__ __forceinline__ double face_u(const float* U, const float* cu, const DevParams& p, int fx, int y, int z) {
+    if (fx == 0 || fx == p.nx) return 0.0;
+    double val = U[uidx(p, fx, y, z)];
     val += cu[sidx(p, fx - 1, y, z)];
     val += cu[sidx(p, fx, y, z)];
-    return fminf(4.0f, fmaxf(-4.0f, val));
+    return fmin(4.0, fmax(-4.0, val));
 }
-__device__ __forceinline__ float face_v(const float* V, const float* cv, const DevParams& p, int x, int fy, int z) {
-    if (fy == 0) return 0.0f;
-    float val = V[vidx(p, x, fy, z)];
+__device__ __forceinline__ double face_v(const float* V, const float* cv, const DevParams& p, int x, int fy, int z) {
+    if (fy == 0) return 0.0;
+    double val = V[vidx(p, x, fy, z)];
     val += cv[sidx(p, x, fy - 1, z)];
-    return fminf(6.4f, fmaxf(-0.9f, val));
+    return fmin(static_cast<double>(6.4f), fmax(static_cast<double>(-0.9f), val));  // the reference's float bounds
 }
-__device__ __forceinline__ float face_w(const float* W, const float* cw, const DevParams& p, int x, int y, int fz) {
-    if (fz == 0 || fz == p.nz) return 0.0f;
-    float val = W[widx(p, x, y, fz)];
+__device__ __forceinline__ double face_w(const float* W, const float* cw, const DevParams& p, int x, int y, int fz) {
+    if (fz == 0 || fz == p.nz) return 0.0;
+    double val = W[widx(p, x, y, fz)];
     val += cw[sidx(p, x, y, fz - 1)];
     val += cw[sidx(p, x, y, fz)];
-    return fminf(4.0f, fmaxf(-4.0f, val));
+    return fmin(4.0, fmax(-4.0, val));
 }
 
 __device__ __forceinline__ void force_apply_at(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
@@ -391,16 +396,16 @@ __device__ __forceinline__ void force_apply_at(float* __restrict__ uo, float* __
                                                const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
                                                float* __restrict__ D, float* __restrict__ P, const DevParams& p, int x, int y, int z) {
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
-    const float uL = face_u(U, cu, p, x, y, z), uR = face_u(U, cu, p, x + 1, y, z);
-    const float vD = face_v(V, cv, p, x, y, z), vU = face_v(V, cv, p, x, y + 1, z);
-    const float wB = face_w(W, cw, p, x, y, z), wF = face_w(W, cw, p, x, y, z + 1);
-    uo[uidx(p, x, y, z)] = uL;
-    if (x == p.nx - 1) uo[uidx(p, x + 1, y, z)] = uR;
-    vo[vidx(p, x, y, z)] = vD;
-    if (y == p.ny - 1) vo[vidx(p, x, y + 1, z)] = vU;
-    wo[widx(p, x, y, z)] = wB;
-    if (z == p.nz - 1) wo[widx(p, x, y, z + 1)] = wF;
-    const float div = uR - uL + vU - vD + wF - wB;
+    const double uL = face_u(U, cu, p, x, y, z), uR = face_u(U, cu, p, x + 1, y, z);
+    const double vD = face_v(V, cv, p, x, y, z), vU = face_v(V, cv, p, x, y + 1, z);
+    const double wB = face_w(W, cw, p, x, y, z), wF = face_w(W, cw, p, x, y, z + 1);
+    uo[uidx(p, x, y, z)] = static_cast<float>(uL);
+    if (x == p.nx - 1) uo[uidx(p, x + 1, y, z)] = static_cast<float>(uR);
+    vo[vidx(p, x, y, z)] = static_cast<float>(vD);
+    if (y == p.ny - 1) vo[vidx(p, x, y + 1, z)] = static_cast<float>(vU);
+    wo[widx(p, x, y, z)] = static_cast<float>(wB);
+    if (z == p.nz - 1) wo[widx(p, x, y, z + 1)] = static_cast<float>(wF);
+    const float div = static_cast<float>(uR - uL + vU - vD + wF - wB);
     const size_t idx = sidx(p, x, y, z);
     D[idx] = -div;
     P[idx] = 0.0f;
