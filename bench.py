@@ -185,13 +185,13 @@ def bench_batched(seed: int, steps: int, count: int):
     return result
 
 
-def bench_served(seed: int, count: int, rate: float, budget_gb: float, storage: int):
+def bench_served(seed: int, count: int, rate: float, budget_gb: float, storage: int, use_graph: bool = True):
     """Served stream: Poisson requests through fs_server (continuous batching,
     HBM-budget admission, a frame every 10 steps), then the same requests
     serially through the plain FP32 path."""
     reqs = workloads.served_requests(seed, count, rate)
     total_voxel_steps = sum(math.prod(r["grid"]) * r["steps"] for r in reqs)
-    with firesim.Server(hbm_budget_bytes=int(budget_gb * (1 << 30)), max_active=256, sor_kernel=-1) as sv:
+    with firesim.Server(hbm_budget_bytes=int(budget_gb * (1 << 30)), max_active=256, sor_kernel=-1, use_graph=use_graph) as sv:
         for r in reqs:
             sv.submit(firesim.make_request(*r["grid"], r["params"], r["emitters"], steps=r["steps"], render_every=r["render_every"],
                                            frame=r["frame"], storage=storage, arrival_s=r["arrival"]))
@@ -249,6 +249,7 @@ def main():
     ap.add_argument("--served-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--served-budget-gb", type=float, default=200.0)
     ap.add_argument("--served-fp32", action="store_true", help="serve FP32 fields instead of BF16 heat/fuel/soot")
+    ap.add_argument("--served-no-graph", action="store_true", help="direct launches instead of captured batch step graphs")
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-scene uneven batch")
     ap.add_argument("--batched-scenes", type=int, default=48)
     ap.add_argument("--batched-steps", type=int, default=10)
@@ -267,7 +268,8 @@ def main():
         return
     if args.only == "served":
         storage = firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16
-        print(json.dumps({"served": bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb, storage)}))
+        print(json.dumps({"served": bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb, storage,
+                                                 not args.served_no_graph)}))
         return
     if args.only == "single":
         args.no_batched = args.no_served = True
@@ -333,7 +335,8 @@ def main():
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.config, args.seed, args.cpu_budget)
     batched = None if args.no_batched else bench_batched(args.seed, args.batched_steps, args.batched_scenes)
     served = None if args.no_served else bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb,
-                                                      firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16)
+                                                      firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16,
+                                                      not args.served_no_graph)
     out = {
         "metric": "voxel-steps/s at 128³ + served p50/p99 step latency vs plain FP32 path",
         "value": value,

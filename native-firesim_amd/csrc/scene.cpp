@@ -374,11 +374,7 @@ bool narrow_field(const fs_scene* s, int f) {
 
 }  // namespace
 
-namespace fsi {
-DevParams dev_params(const fs_scene& sc) { return ::dev_params(sc); }
-void drop_graphs(fs_scene* s) { ::drop_graphs(s); }
-int global_fail(int status, const std::string& msg) { return ::global_fail(status, msg); }
-}  // namespace fsi
+
 
 // ====================================================================== ABI
 extern "C" {
@@ -439,7 +435,7 @@ namespace {
 
 // fs_create / fs_create_in: `memory` is null (the scene allocates its pool)
 // or caller-owned device memory of at least total_bytes(dims) bytes.
-int create_scene(const fs_scene_desc* desc, void* memory, uint64_t memoryBytes, fs_scene** out) {
+int create_scene(const fs_scene_desc* desc, void* memory, uint64_t memoryBytes, fs_scene** out, bool reset = true) {
     std::string why;
     if (!out) return global_fail(FS_ERR_INVALID_ARGUMENT, "null output handle");
     *out = nullptr;
@@ -505,7 +501,7 @@ int create_scene(const fs_scene_desc* desc, void* memory, uint64_t memoryBytes, 
     ds.metricsOut = static_cast<double*>(take());
     ds.clock = static_cast<float*>(take());
     fs_default_params(&s->prm);
-    const int st = fs_reset(s);
+    const int st = reset ? fs_reset(s) : FS_OK;
     if (st != FS_OK) {
         g_globalError = s->lastError;
         fs_destroy(s);
@@ -539,20 +535,9 @@ int fs_destroy(fs_scene* s) {
 // resetSimulationFields src/fire_cuda.cu:3006-3060: all fields to their
 // initial state (the bed evaluated at time 0), light and shadow zero, clocks 0.
 int fs_reset(fs_scene* s) {
-    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
-    if (int st = set_device(s)) return st;
-    // Everything except the scalars is zero at reset; clear the whole pool.
-    hipError_t e = hipMemsetAsync(s->pool, 0, s->poolBytes, s->stream);
-    if (e != hipSuccess) return s->hip_fail("reset memset", e);
-    const DevParams p = dev_params(*s);
-    e = fs::launch_reset(s->ds, p, s->stream);
-    if (e != hipSuccess) return s->hip_fail("reset kernel", e);
-    e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) return s->hip_fail("reset sync", e);
-    s->time = 0.0f;
-    s->renderTime = 0.0f;
-    s->frameIndex = 0;
-    return FS_OK;
+    if (int st = fsi::reset_async(s)) return st;
+    const hipError_t e = hipStreamSynchronize(s->stream);
+    return e == hipSuccess ? FS_OK : s->hip_fail("reset sync", e);
 }
 
 int fs_set_params(fs_scene* s, const fs_params* prm) {
@@ -935,3 +920,28 @@ int fs_metrics_collect(fs_scene* s, fs_metrics* m) {
 void* fs_stream(fs_scene* s) { return s ? static_cast<void*>(s->stream) : nullptr; }
 
 }  // extern "C"
+
+namespace fsi {
+int create_unreset(const fs_scene_desc* desc, void* memory, uint64_t bytes, fs_scene** out) {
+    return create_scene(desc, memory, bytes, out, false);
+}
+
+int reset_async(fs_scene* s) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (int st = set_device(s)) return st;
+    // Everything except the scalars is zero at reset; clear the whole pool.
+    hipError_t e = hipMemsetAsync(s->pool, 0, s->poolBytes, s->stream);
+    if (e != hipSuccess) return s->hip_fail("reset memset", e);
+    const DevParams p = dev_params(*s);
+    e = fs::launch_reset(s->ds, p, s->stream);
+    if (e != hipSuccess) return s->hip_fail("reset kernel", e);
+    s->time = 0.0f;
+    s->renderTime = 0.0f;
+    s->frameIndex = 0;
+    return FS_OK;
+}
+
+DevParams dev_params(const fs_scene& sc) { return ::dev_params(sc); }
+void drop_graphs(fs_scene* s) { ::drop_graphs(s); }
+int global_fail(int status, const std::string& msg) { return ::global_fail(status, msg); }
+}  // namespace fsi

@@ -59,6 +59,11 @@ struct fs_scene {
 };
 
 namespace fsi {
+// fs_create / fs_create_in (memory may be null) without the initial reset:
+// the fields hold garbage until reset_async or fs_reset.
+int create_unreset(const fs_scene_desc* desc, void* memory, uint64_t bytes, fs_scene** out);
+// fs_reset without the final wait: enqueued on the scene's current stream.
+int reset_async(fs_scene* s);
 // Kernel parameter block of a scene's current parameters and emitters.
 DevParams dev_params(const fs_scene& sc);
 // Releases the scene's captured step graphs (they are recaptured on demand).

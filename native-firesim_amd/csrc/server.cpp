@@ -46,6 +46,7 @@ struct Request {
     uint32_t* frame[kFrameBuffers] = {nullptr, nullptr, nullptr};
     int frameStep = -1;  // step index of the newest complete frame
     int frameBuf = -1;
+    std::vector<uint32_t> finalFrame;  // newest frame, kept after the pinned buffers are recycled
     double arrival = 0.0, admitted = -1.0, finished = -1.0;
     double readyAt = 0.0;
     std::vector<double> lat;
@@ -82,6 +83,7 @@ struct fs_server {
     std::deque<Round> inflight;
     std::vector<hipEvent_t> eventPool;
     std::multimap<uint64_t, void*> cache;     // recycled device blocks by size
+    std::multimap<size_t, uint32_t*> framePool;  // recycled pinned frame buffers by size
     uint64_t cachedBytes = 0;
     uint64_t usedBytes = 0;
     uint64_t peakBytes = 0;
@@ -156,9 +158,28 @@ void put_block(fs_server* sv, void* p, uint64_t bytes) {
     sv->cachedBytes += bytes;
 }
 
-void free_frames(Request& q) {
+size_t frame_bytes(const Request& q) { return static_cast<size_t>(q.r.desc.frame_w) * q.r.desc.frame_h * 4; }
+
+// Pinned frame buffers come from a pool: hipHostMalloc is slow and would
+// stall the scheduling loop at every admission.
+bool get_frames(fs_server* sv, Request& q) {
+    const size_t bytes = frame_bytes(q);
     for (auto& f : q.frame) {
-        if (f) (void)hipHostFree(f);
+        auto it = sv->framePool.find(bytes);
+        if (it != sv->framePool.end()) {
+            f = it->second;
+            sv->framePool.erase(it);
+        } else if (hipHostMalloc(reinterpret_cast<void**>(&f), bytes, hipHostMallocDefault) != hipSuccess) {
+            f = nullptr;
+            return false;
+        }
+    }
+    return true;
+}
+
+void put_frames(fs_server* sv, Request& q) {
+    for (auto& f : q.frame) {
+        if (f) sv->framePool.emplace(frame_bytes(q), f);
         f = nullptr;
     }
 }
@@ -169,29 +190,28 @@ bool admit(fs_server* sv, Request& q, double t) {
     d.device = sv->desc.device;
     if (static_cast<int>(sv->active.size() + sv->draining.size()) >= sv->desc.max_active) return false;
     if (sv->usedBytes + q.sceneBytes > sv->desc.hbm_budget_bytes) return false;
+    // Nothing here waits for the device: the scene is created without its
+    // initial reset, joins the batch, and is reset on the batch stream, in
+    // order before its first step.
     int st = FS_OK;
     if (q.r.keep_scene) {
-        st = fs_create(&d, &q.scene);
+        st = fsi::create_unreset(&d, nullptr, 0, &q.scene);
     } else {
         q.memory = get_block(sv, q.sceneBytes, &q.memoryBytes);
-        st = q.memory ? fs_create_in(&d, q.memory, q.memoryBytes, &q.scene) : FS_ERR_OUT_OF_MEMORY;
+        st = q.memory ? fsi::create_unreset(&d, q.memory, q.memoryBytes, &q.scene) : FS_ERR_OUT_OF_MEMORY;
     }
     if (st == FS_OK) st = fs_set_params(q.scene, &q.r.params);
     if (st == FS_OK) st = fs_set_emitters(q.scene, q.r.emitters, q.r.emitter_count);
-    if (st == FS_OK) st = fs_reset(q.scene);
-    if (st == FS_OK && q.r.render_every > 0) {
-        const size_t bytes = static_cast<size_t>(d.frame_w) * d.frame_h * 4;
-        for (auto& f : q.frame)
-            if (st == FS_OK && hipHostMalloc(reinterpret_cast<void**>(&f), bytes, hipHostMallocDefault) != hipSuccess) st = FS_ERR_OUT_OF_MEMORY;
-    }
+    if (st == FS_OK && q.r.render_every > 0 && !get_frames(sv, q)) st = FS_ERR_OUT_OF_MEMORY;
     if (st == FS_OK) st = fs_batch_add(sv->batch, q.scene);
+    if (st == FS_OK) st = fsi::reset_async(q.scene);
     if (st != FS_OK) {
         q.error = q.scene ? fs_last_error(q.scene) : fs_last_global_error();
         if (q.scene) fs_destroy(q.scene);
         q.scene = nullptr;
         put_block(sv, q.memory, q.memoryBytes);
         q.memory = nullptr;
-        free_frames(q);
+        put_frames(sv, q);
         q.state = FS_REQ_FAILED;
         q.finished = t;
         ++sv->failed;
@@ -211,6 +231,11 @@ void retire(fs_server* sv, Request& q, double t) {
     q.state = FS_REQ_DONE;
     q.finished = t;
     ++sv->done;
+    if (q.frameBuf >= 0) {
+        const uint32_t* f = q.frame[q.frameBuf];
+        q.finalFrame.assign(f, f + frame_bytes(q) / 4);
+    }
+    put_frames(sv, q);
     if (!q.r.keep_scene) {
         fs_destroy(q.scene);  // its work completed: no wait
         q.scene = nullptr;
@@ -304,8 +329,10 @@ int fs_server_destroy(fs_server* sv) {
         if (q.scene) fs_destroy(q.scene);
         q.scene = nullptr;
         if (q.memory) (void)hipFree(q.memory);
-        free_frames(q);
+        for (auto& f : q.frame)
+            if (f) (void)hipHostFree(f);
     }
+    for (auto& kv : sv->framePool) (void)hipHostFree(kv.second);
     for (auto& rd : sv->inflight)
         if (rd.done) (void)hipEventDestroy(rd.done);
     for (hipEvent_t e : sv->eventPool) (void)hipEventDestroy(e);
@@ -359,7 +386,10 @@ int fs_server_tick(fs_server* sv, int32_t* activeOut) {
     }
     while (!sv->waiting.empty() && admit(sv, sv->reqs[sv->waiting.front()], t)) sv->waiting.pop_front();
     if (sv->active.empty()) return FS_OK;
-    if (!sv->tuned && (sv->roundsStepped >= 8 || static_cast<int>(sv->active.size()) >= std::min(sv->desc.max_active, 8))) {
+    uint64_t activeCells = 0;
+    for (uint64_t id : sv->active) activeCells += sv->reqs[id].cells;
+    // tune on a representative mix: once 16M cells are resident (or after 64 rounds)
+    if (!sv->tuned && (activeCells >= (16ull << 20) || sv->roundsStepped >= 64)) {
         fs_autotune_report rep;
         if (int st = fs_batch_autotune(sv->batch, 1, &rep)) return sv->fail(st, fs_batch_last_error(sv->batch));
         sv->tuned = true;
@@ -451,8 +481,9 @@ int fs_server_frame(fs_server* sv, uint64_t id, uint32_t* bgra, int32_t* frameSt
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) return sv->fail(FS_ERR_INVALID_ARGUMENT, "unknown request id");
     const Request& q = it->second;
-    if (q.frameBuf < 0 || !q.frame[q.frameBuf]) return sv->fail(FS_ERR_INVALID_ARGUMENT, "no frame rendered yet");
-    if (bgra) std::memcpy(bgra, q.frame[q.frameBuf], static_cast<size_t>(q.r.desc.frame_w) * q.r.desc.frame_h * 4);
+    if (q.frameBuf < 0) return sv->fail(FS_ERR_INVALID_ARGUMENT, "no frame rendered yet");
+    const uint32_t* src = q.frame[q.frameBuf] ? q.frame[q.frameBuf] : q.finalFrame.data();
+    if (bgra) std::memcpy(bgra, src, frame_bytes(q));
     if (frameStep) *frameStep = q.frameStep;
     return FS_OK;
 }
