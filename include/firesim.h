@@ -286,6 +286,24 @@ int fs_metrics_collect(fs_scene* scene, fs_metrics* out);
 /* Device stream of the scene (hipStream_t) for event timing by the caller. */
 void* fs_stream(fs_scene* scene);
 
+/* ---- reference scenes and frame statistics (host only) ---- */
+/* Emitter and burner placement of reference scene scene_id (0 room, 1
+ * campfire, 2 gas burner), read from <assets_dir>/<scene>/scene.json and
+ * emitter-mask.json as the reference does (loadSceneEmitterParams
+ * src/main.cpp:1397-1476), written into the emitter_* and burner_* fields of
+ * *params (applySceneEmitterParams :433-447). Missing files give the
+ * reference's built-in values. assets_dir null: "assets/fire-scenes". */
+int fs_scene_emitter_params(int32_t scene_id, const char* assets_dir, fs_params* params);
+
+/* Colour statistics of a BGRA frame (computeImageStats src/main.cpp:2635-2690). */
+typedef struct fs_image_stats {
+    float mean_luma, max_luma, mean_saturation;
+    float warm_fire_fraction, tan_smoke_fraction, black_smoke_fraction, white_core_fraction;
+    float bright_fraction;
+    int32_t bright_pixels;
+} fs_image_stats;
+int fs_image_stats_of(const uint32_t* bgra, int64_t pixels, fs_image_stats* out);
+
 /* ---- batches: many scenes, one launch per pass ----
  * No counterpart in the reference, which has one simulation per process
  * (src/fire_cuda.cu:116-195). A batch steps every member scene once per

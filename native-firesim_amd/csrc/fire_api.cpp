@@ -279,3 +279,37 @@ void fireCudaShutdown() {
 const char* fireCudaLastError() { return g_lastError; }
 
 fs_scene* fireCudaScene() { return g_scene; }
+
+void fireApplyCanonicalSettings(FireSettings* s) {
+    if (!s) return;
+    s->sceneId = std::max(0, std::min(2, s->sceneId));
+    s->cinematicMode = 1;
+    s->raymarchSteps = 104;
+    s->emberCount = 176;
+    s->exposure = 0.84f;
+    s->reflectionGain = 0.54f;
+    s->smokeDarkness = 1.26f;
+    s->intensity = 1.42f;
+    s->smoke = 0.96f;
+    s->turbulence = 1.34f;
+    s->renderDebugMode = 0;
+}
+
+bool fireApplySceneEmitter(FireSettings* s, const char* assetsDir) {
+    if (!s) return fail("Settings pointer was null.");
+    fs_params p;
+    fs_default_params(&p);
+    if (fs_scene_emitter_params(std::max(0, std::min(2, s->sceneId)), assetsDir, &p) != FS_OK) return fail("Unknown scene id.");
+    s->emitterCenterX = p.emitter_center_x;
+    s->emitterCenterZ = p.emitter_center_z;
+    s->emitterHeightNorm = p.emitter_height_norm;
+    s->emitterHeightBandNorm = p.emitter_height_band_norm;
+    s->emitterRadius = p.emitter_radius;
+    s->burnerCenterCount = p.burner_center_count;
+    for (int i = 0; i < 4; ++i) {
+        s->burnerCenterX[i] = p.burner_center_x[i];
+        s->burnerCenterY[i] = p.burner_center_y[i];
+        s->burnerCenterZ[i] = p.burner_center_z[i];
+    }
+    return true;
+}

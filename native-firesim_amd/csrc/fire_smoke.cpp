@@ -2,8 +2,9 @@
 // src/main.cpp:4170-4218) driven through fire_api.h: initialise for a
 // 960x540 frame and the canonical 384x240 grid request (176x208x128 cells),
 // run 64 step+render frames with the same per-frame settings, write the last
-// frame as a BMP. Usage: fire_smoke <output.bmp>. Exit codes as the
-// reference: 0 ok, 2 initialisation failed, 3 a frame or the BMP failed.
+// frame as a BMP. Usage: fire_smoke <output.bmp> [assets dir]. Exit codes as
+// the reference: 0 ok, 2 initialisation failed, 3 a frame or the BMP failed.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -13,6 +14,7 @@
 
 int main(int argc, char** argv) {
     const char* out = argc > 1 ? argv[1] : "fire-smoke-test-frame.bmp";
+    const char* assets = argc > 2 ? argv[2] : nullptr;
     constexpr int kFrameWidth = 960, kFrameHeight = 540;
     std::vector<std::uint32_t> frame(static_cast<size_t>(kFrameWidth) * kFrameHeight, 0xff000000u);
     if (!fireCudaInitialize(kFrameWidth, kFrameHeight, 384, 240)) {
@@ -30,16 +32,11 @@ int main(int argc, char** argv) {
         s.mouseY = 0.13f + 0.04f * std::sin(static_cast<float>(i) * 0.045f);
         s.wind = 0.08f;
         s.detail = 0.92f;
-        // canonical settings (src/main.cpp:2758-2770 with the constants at :37-44)
-        s.cinematicMode = 1;
-        s.raymarchSteps = 104;
-        s.emberCount = 176;
-        s.exposure = 0.84f;
-        s.reflectionGain = 0.54f;
-        s.smokeDarkness = 1.26f;
-        s.intensity = 1.42f;
-        s.smoke = 1.05f;
-        s.turbulence = 1.34f;
+        fireApplyCanonicalSettings(&s);
+        fireApplySceneEmitter(&s, assets);
+        s.turbulence = std::max(s.turbulence, 1.08f);
+        s.smoke = std::max(s.smoke, 1.05f);
+        s.intensity = std::max(s.intensity, 1.05f);
         s.cameraYaw = 0.18f;
         s.cameraPitch = 0.08f;
         s.cameraDistance = 2.62f;

@@ -134,3 +134,12 @@ const char* fireCudaLastError();
 // for callers that mix this interface with the per-scene C ABI.
 struct fs_scene;
 fs_scene* fireCudaScene();
+
+// Host helpers the reference keeps in its application (src/main.cpp), for
+// drivers built on this interface:
+// applyCanonicalFireSettings (:2758-2770) with the constants at :37-44.
+void fireApplyCanonicalSettings(FireSettings* settings);
+// loadSceneEmitterParams + applySceneEmitterParams (:1397-1476, :433-447) for
+// settings->sceneId, reading <assetsDir>/<scene>/scene.json and
+// emitter-mask.json (null: "assets/fire-scenes"); false for a bad scene id.
+bool fireApplySceneEmitter(FireSettings* settings, const char* assetsDir);

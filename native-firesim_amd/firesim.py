@@ -129,6 +129,15 @@ class ServerStats(Structure):
 REQ_PENDING, REQ_WAITING, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(5)
 
 
+class ImageStats(Structure):
+    _fields_ = [("mean_luma", c_float), ("max_luma", c_float), ("mean_saturation", c_float),
+                ("warm_fire_fraction", c_float), ("tan_smoke_fraction", c_float), ("black_smoke_fraction", c_float),
+                ("white_core_fraction", c_float), ("bright_fraction", c_float), ("bright_pixels", c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class Tuning(Structure):
     _fields_ = [("sor_kernel", c_int32), ("use_graph", c_int32)]
 
@@ -267,6 +276,8 @@ def load_library(path: str = LIB_PATH):
         "fs_batch_last_error": (c_char_p, [c_void_p]),
         "fs_batch_stream": (c_void_p, [c_void_p]),
         "fs_create_in": (c_int32, [P(SceneDesc), c_void_p, c_uint64, P(c_void_p)]),
+        "fs_scene_emitter_params": (c_int32, [c_int32, c_char_p, P(Params)]),
+        "fs_image_stats_of": (c_int32, [P(c_uint32), c_int64, P(ImageStats)]),
         "fs_render_bgra8_async": (c_int32, [c_void_p, P(c_uint32)]),
         "fs_server_create": (c_int32, [P(ServerDesc), P(c_void_p)]),
         "fs_server_destroy": (c_int32, [c_void_p]),
@@ -650,3 +661,22 @@ class Server:
 
     def clock(self) -> float:
         return self.lib.fs_server_clock(self.h)
+
+
+def scene_emitter_params(scene_id: int, assets_dir: str | None = None, params: Params | None = None) -> Params:
+    """Emitter/burner placement of a reference scene (fs_scene_emitter_params) applied to params."""
+    p = params if params is not None else default_params(scene_id=scene_id)
+    st = load_library().fs_scene_emitter_params(scene_id, assets_dir.encode() if assets_dir else None, byref(p))
+    if st != FS_OK:
+        raise FireSimError(f"fs_scene_emitter_params({scene_id}) failed ({st})")
+    return p
+
+
+def image_stats(bgra: np.ndarray) -> dict:
+    """Colour statistics of a BGRA frame (fs_image_stats_of)."""
+    a = np.ascontiguousarray(bgra, dtype=np.uint32).ravel()
+    out = ImageStats()
+    st = load_library().fs_image_stats_of(a.ctypes.data_as(POINTER(c_uint32)), a.size, byref(out))
+    if st != FS_OK:
+        raise FireSimError(f"fs_image_stats_of failed ({st})")
+    return out.as_dict()
