@@ -190,15 +190,16 @@ typedef struct fs_stage_times {
 /* Pressure-solve kernels. All give bit-identical results (same per-cell FP32
  * expression in the same red/black order); they differ in how many
  * half-sweeps one launch performs from an LDS-resident tile of p (d read from
- * global memory). */
+ * global memory) and in the tile shape: x/y edge T, z depth TZ, K half-sweeps
+ * per launch on a (T+2K)^2 x (TZ+2K) box. fs_autotune picks per grid. */
 enum fs_sor_kernel {
-    FS_SOR_HALF_SWEEP = 0,   /* one launch per half-sweep, as the reference (:3280-3284) */
-    FS_SOR_TB_T20_K2 = 1,    /* 20^3 tiles, 2 half-sweeps per launch */
-    FS_SOR_TB_T16_K4 = 2,    /* 16^3 tiles, 4 half-sweeps per launch (default) */
-    FS_SOR_TB_T8_K4 = 3,     /* 8^3 tiles, 4 half-sweeps per launch */
-    FS_SOR_TB_T12_K6 = 4,    /* 12^3 tiles, 6 half-sweeps per launch */
-    FS_SOR_TB_T8_K8 = 5,     /* 8^3 tiles, 8 half-sweeps per launch */
-    FS_SOR_TB_T18_K3 = 6,    /* 18^3 tiles, 3 half-sweeps per launch */
+    FS_SOR_HALF_SWEEP = 0,      /* one launch per half-sweep, as the reference (:3280-3284) */
+    FS_SOR_TB_T16_K4 = 1,       /* 16^3 tiles, 4 half-sweeps per launch */
+    FS_SOR_TB_T24_K4 = 2,       /* 24^3 tiles, 4 half-sweeps per launch (default) */
+    FS_SOR_TB_T24_Z8_K4 = 3,    /* 24x24x8 tiles */
+    FS_SOR_TB_T24_Z16_K4 = 4,   /* 24x24x16 tiles */
+    FS_SOR_TB_T24_Z28_K4 = 5,   /* 24x24x28 tiles */
+    FS_SOR_TB_T24_Z10_K4 = 6,   /* 24x24x10 tiles */
     FS_SOR_KERNEL_COUNT = 7
 };
 

@@ -153,5 +153,6 @@ int pressure_launch_count(const DeviceScene& s, int iterations);
 // Half-sweeps one temporally blocked SOR launch performs (1 for the plain kernel).
 int sor_depth(int sorKernel);
 int sor_tile_edge(int sorKernel);
+int sor_tile_depth(int sorKernel);
 
 }  // namespace fs

@@ -465,35 +465,38 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
 // along its planes the colour parity of its x position is fixed, so the wall
 // clamps in x and y become fixed LDS offsets (offset 0 reads the cell itself).
 // d is read from global memory (L2) inside the sweep, which keeps the kernel
-// at ~31 VGPRs and 55 KB of LDS so that two workgroups share a CU.
-template <int T, int K>
+// near 35 VGPRs. The tile shapes trade halo redundancy against occupancy:
+// 16^3 tiles (55 KB of LDS, two workgroups per CU) suit small grids, 24x24
+// tiles with a 32x32 loaded face (1024 threads, one or two per CU) larger ones.
+template <int T, int K, int TZ>
 __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D, float omega, int h0,
                                          int count, int nx, int ny, int nz, float* __restrict__ sp, uint3 blk) {
-    constexpr int E = T + 2 * K;
+    constexpr int E = T + 2 * K;    // loaded edge in x and y
+    constexpr int EZ = TZ + 2 * K;  // loaded depth in z
     constexpr int NT = E * E;  // threads per workgroup
     constexpr int HX = E / 2;  // cells of one colour per LDS row
     static_assert(E % 2 == 0 && NT % 64 == 0, "tile edge plus halo must be even and E^2 a multiple of 64");
     const int tid = threadIdx.x;
-    const int ox = blk.x * T, oy = blk.y * T, oz = blk.z * T;
+    const int ox = blk.x * T, oy = blk.y * T, oz = blk.z * TZ;
     const int bx = ox - K, by = oy - K, bz = oz - K;  // global coordinate of local 0
     const size_t row = static_cast<size_t>(nx), plane = row * ny;
     {
         const int gx = bx + tid % E, gy = by + tid / E;
         const bool colOk = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-        float v[E];
+        float v[EZ];
         const float* src = P + (colOk ? gy * row + gx : 0);
 #pragma unroll
-        for (int lz = 0; lz < E; ++lz) {
+        for (int lz = 0; lz < EZ; ++lz) {
             const int gz = bz + lz;
             v[lz] = (colOk && gz >= 0 && gz < nz) ? src[gz * plane] : 0.0f;
         }
 #pragma unroll
-        for (int lz = 0; lz < E; ++lz) sp[lz * NT + tid] = v[lz];
+        for (int lz = 0; lz < EZ; ++lz) sp[lz * NT + tid] = v[lz];
     }
     __syncthreads();
     // local index range of real cells
     const int rx0 = max(0, -bx), ry0 = max(0, -by), rz0 = max(0, -bz);
-    const int rx1 = min(E, nx - bx), ry1 = min(E, ny - by), rz1 = min(E, nz - bz);
+    const int rx1 = min(E, nx - bx), ry1 = min(E, ny - by), rz1 = min(EZ, nz - bz);
     const int k = tid % HX, ly = (tid / HX) % E, zg = tid / (HX * E);
     const int gy = by + ly;
     const int oD = gy > 0 ? -E : 0, oU = gy < ny - 1 ? E : 0;
@@ -502,7 +505,7 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
         const int ax = bx < 0 ? rx0 : s + 1, ay = by < 0 ? ry0 : s + 1, az = bz < 0 ? rz0 : s + 1;
         const int ex = ox + T + K > nx ? rx1 : E - 1 - s;
         const int ey = oy + T + K > ny ? ry1 : E - 1 - s;
-        const int ez = oz + T + K > nz ? rz1 : E - 1 - s;
+        const int ez = oz + TZ + K > nz ? rz1 : EZ - 1 - s;
         const int lx = 2 * k + ((parity ^ (bx + gy + bz + zg)) & 1);
         if (lx >= ax && lx < ex && ly >= ay && ly < ey) {
             const int gx = bx + lx;
@@ -534,17 +537,17 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
             float* dst = Pout + gy2 * row + gx;
             const int src = (K * E + ty + K) * E + tx + K;
 #pragma unroll
-            for (int tz = 0; tz < T; ++tz)
+            for (int tz = 0; tz < TZ; ++tz)
                 if (oz + tz < nz) dst[(oz + tz) * plane] = sp[src + tz * NT];
         }
     }
 }
 
-template <int T, int K>
+template <int T, int K, int TZ>
 __global__ __launch_bounds__((T + 2 * K) * (T + 2 * K)) void k_sor_tb(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D,
                                                                      float omega, int h0, int count, int nx, int ny, int nz) {
-    __shared__ float sp[(T + 2 * K) * (T + 2 * K) * (T + 2 * K)];
-    sor_tile<T, K>(P, Pout, D, omega, h0, count, nx, ny, nz, sp, xcd_block());
+    __shared__ float sp[(T + 2 * K) * (T + 2 * K) * (TZ + 2 * K)];
+    sor_tile<T, K, TZ>(P, Pout, D, omega, h0, count, nx, ny, nz, sp, xcd_block());
 }
 
 // ----------------------------------------------------------- K13-K14 ----
@@ -807,15 +810,15 @@ __global__ __launch_bounds__(256) void kb_force_apply(const BatchTable* __restri
 }
 
 // Pressure launch j: half-sweeps j*K .. j*K+K-1 of every scene that has them.
-template <int T_, int K>
+template <int T_, int K, int TZ>
 __global__ __launch_bounds__((T_ + 2 * K) * (T_ + 2 * K)) void kb_sor_tb(const BatchTable* __restrict__ T, int j) {
-    __shared__ float sp[(T_ + 2 * K) * (T_ + 2 * K) * (T_ + 2 * K)];
+    __shared__ float sp[(T_ + 2 * K) * (T_ + 2 * K) * (TZ + 2 * K)];
     const int h0 = j * K;
     batch_blocks(T->sorStart, T->sorActive[j], [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
         const float* in = (j & 1) ? S.p1 : S.p0;
         float* out = (j & 1) ? S.p0 : S.p1;
-        sor_tile<T_, K>(in, out, S.d, S.p.omega, h0, min(K, S.halfSweeps - h0), S.p.nx, S.p.ny, S.p.nz, sp, block_of(local, S.sorG));
+        sor_tile<T_, K, TZ>(in, out, S.d, S.p.omega, h0, min(K, S.halfSweeps - h0), S.p.nx, S.p.ny, S.p.nz, sp, block_of(local, S.sorG));
         __syncthreads();  // the next tile reuses sp
     });
 }
@@ -886,52 +889,48 @@ hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st) {
     return hipGetLastError();
 }
 
-int sor_depth(int sorKernel) {
+// Temporally blocked SOR kernels by enum fs_sor_kernel: x/y tile edge T,
+// z depth TZ, half-sweeps per launch K (the loaded box is (T+2K)^2 x (TZ+2K)).
+struct SorShape {
+    int t, tz, k;
+};
+static SorShape sor_shape(int sorKernel) {
     switch (sorKernel) {
-        case 1: return 2;  // T20 K2
-        case 2: return 4;  // T16 K4
-        case 3: return 4;  // T8 K4
-        case 4: return 6;  // T12 K6
-        case 5: return 8;  // T8 K8
-        case 6: return 3;  // T18 K3
-        default: return 1;
+        case 1: return {16, 16, 4};
+        case 2: return {24, 24, 4};
+        case 3: return {24, 8, 4};
+        case 4: return {24, 16, 4};
+        case 5: return {24, 28, 4};
+        case 6: return {24, 10, 4};
+        default: return {0, 0, 1};  // plain half-sweep kernel
     }
 }
 
-// Tile edge of a temporally blocked SOR kernel (0 for the plain half-sweep kernel).
-int sor_tile_edge(int sorKernel) {
-    switch (sorKernel) {
-        case 1: return 20;
-        case 2: return 16;
-        case 3: return 8;
-        case 4: return 12;
-        case 5: return 8;
-        case 6: return 18;
-        default: return 0;
-    }
-}
+int sor_depth(int sorKernel) { return sor_shape(sorKernel).k; }
+int sor_tile_edge(int sorKernel) { return sor_shape(sorKernel).t; }
+int sor_tile_depth(int sorKernel) { return sor_shape(sorKernel).tz; }
 
-template <int T, int K>
+template <int T, int K, int TZ = T>
 static void launch_sor_tb(DeviceScene& s, const DevParams& p, hipStream_t st) {
-    const dim3 g((s.nx + T - 1) / T, (s.ny + T - 1) / T, (s.nz + T - 1) / T);
+    const dim3 g((s.nx + T - 1) / T, (s.ny + T - 1) / T, (s.nz + TZ - 1) / TZ);
     const int half = 2 * p.iterations;
     constexpr int E = T + 2 * K;
     // Tiles read the halo of their neighbours, so a launch must not overwrite
     // its own input: each launch reads one pressure buffer and writes the other.
     for (int h0 = 0; h0 < half; h0 += K) {
-        k_sor_tb<T, K><<<g, E * E, 0, st>>>(s.p, s.p2, s.d, p.omega, h0, min(K, half - h0), s.nx, s.ny, s.nz);
+        k_sor_tb<T, K, TZ><<<g, E * E, 0, st>>>(s.p, s.p2, s.d, p.omega, h0, min(K, half - h0), s.nx, s.ny, s.nz);
         std::swap(s.p, s.p2);
     }
 }
 
 hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
     switch (s.sorKernel) {
-        case 1: launch_sor_tb<20, 2>(s, p, st); break;
-        case 2: launch_sor_tb<16, 4>(s, p, st); break;
-        case 3: launch_sor_tb<8, 4>(s, p, st); break;
-        case 4: launch_sor_tb<12, 6>(s, p, st); break;
-        case 5: launch_sor_tb<8, 8>(s, p, st); break;
-        case 6: launch_sor_tb<18, 3>(s, p, st); break;
+        case 1: launch_sor_tb<16, 4>(s, p, st); break;
+        case 2: launch_sor_tb<24, 4>(s, p, st); break;
+        case 3: launch_sor_tb<24, 4, 8>(s, p, st); break;
+        case 4: launch_sor_tb<24, 4, 16>(s, p, st); break;
+        case 5: launch_sor_tb<24, 4, 28>(s, p, st); break;
+        case 6: launch_sor_tb<24, 4, 10>(s, p, st); break;
         default: {
             const dim3 b = cell_block();
             const dim3 g = sor_grid(0, s.nx, s.ny, s.nz);
@@ -945,12 +944,12 @@ hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
 }
 
 dim3 sor_grid(int sorKernel, int nx, int ny, int nz) {
-    const int t = sor_tile_edge(sorKernel);
+    const int t = sor_tile_edge(sorKernel), tz = sor_tile_depth(sorKernel);
     if (t == 0) {
         const dim3 b = cell_block();
         return dim3(((nx + 1) / 2 + b.x - 1) / b.x, (ny + b.y - 1) / b.y, (nz + b.z - 1) / b.z);
     }
-    return dim3((nx + t - 1) / t, (ny + t - 1) / t, (nz + t - 1) / t);
+    return dim3((nx + t - 1) / t, (ny + t - 1) / t, (nz + tz - 1) / tz);
 }
 
 int pressure_launch_count(const DeviceScene& s, int iterations) {
@@ -974,11 +973,15 @@ hipError_t launch_metrics(DeviceScene& s, const DevParams& p, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int T, int K>
+template <int T, int K, int TZ = T>
 static void launch_batch_sor_tb(const BatchTable* table, int launches, int grid, hipStream_t st) {
-    constexpr int E = T + 2 * K;
-    // two E^3 LDS tiles fit a CU, so grid / 4 workgroups are resident at once
-    for (int j = 0; j < launches; ++j) kb_sor_tb<T, K><<<grid / 4, E * E, 0, st>>>(table, j);
+    constexpr int E = T + 2 * K, EZ = TZ + 2 * K;
+    // as many workgroups as are resident at once: grid / 8 CUs, each holding
+    // as many LDS tiles (and E^2-thread workgroups) as fit
+    constexpr int byLds = 163840 / (E * E * EZ * 4), byThreads = 2048 / (E * E);
+    constexpr int perCu = byLds < byThreads ? byLds : byThreads;
+    const int wgs = (grid / 8) * (perCu > 0 ? perCu : 1);
+    for (int j = 0; j < launches; ++j) kb_sor_tb<T, K, TZ><<<wgs, E * E, 0, st>>>(table, j);
 }
 
 hipError_t launch_batch_step(const BatchTable* table, bool bf16, int sorKernel, int sorLaunches, int grid, hipStream_t st) {
@@ -1003,12 +1006,12 @@ hipError_t launch_batch_step(const BatchTable* table, bool bf16, int sorKernel, 
 hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sorLaunches, int grid, hipStream_t st) {
     if (grid < 16 || grid % 16 != 0 || sorLaunches < 0 || sorLaunches > kMaxSorLaunches) return hipErrorInvalidValue;
     switch (sorKernel) {
-        case 1: launch_batch_sor_tb<20, 2>(table, sorLaunches, grid, st); break;
-        case 2: launch_batch_sor_tb<16, 4>(table, sorLaunches, grid, st); break;
-        case 3: launch_batch_sor_tb<8, 4>(table, sorLaunches, grid, st); break;
-        case 4: launch_batch_sor_tb<12, 6>(table, sorLaunches, grid, st); break;
-        case 5: launch_batch_sor_tb<8, 8>(table, sorLaunches, grid, st); break;
-        case 6: launch_batch_sor_tb<18, 3>(table, sorLaunches, grid, st); break;
+        case 1: launch_batch_sor_tb<16, 4>(table, sorLaunches, grid, st); break;
+        case 2: launch_batch_sor_tb<24, 4>(table, sorLaunches, grid, st); break;
+        case 3: launch_batch_sor_tb<24, 4, 8>(table, sorLaunches, grid, st); break;
+        case 4: launch_batch_sor_tb<24, 4, 16>(table, sorLaunches, grid, st); break;
+        case 5: launch_batch_sor_tb<24, 4, 28>(table, sorLaunches, grid, st); break;
+        case 6: launch_batch_sor_tb<24, 4, 10>(table, sorLaunches, grid, st); break;
         default:
             for (int j = 0; j < sorLaunches; ++j) kb_sor_half<<<grid, cell_block(), 0, st>>>(table, j);
     }
