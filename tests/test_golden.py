@@ -1,0 +1,65 @@
+"""Regression fixtures in tests/golden (written by tests/golden/make_golden.py
+from the FP64 oracle): the oracle must reproduce them exactly, and the HIP
+path must match them to the step's accuracy target (relative L2 <= 1e-5 per
+field; HDR frame <= 1e-3, as in the parity tests)."""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+import compare
+import firesim
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = sorted(os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+def load_case(name):
+    data = np.load(os.path.join(HERE, "golden", f"{name}.npz"))
+    meta = json.load(open(os.path.join(HERE, "golden", f"{name}.json")))
+    params = firesim.Params.from_buffer_copy(data["params_bytes"].tobytes())
+    return data, meta, params
+
+
+def test_fixtures_present():
+    assert len(CASES) >= 3
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_reproduces_fixture(name):
+    import oracle
+    data, meta, params = load_case(name)
+    o = oracle.OracleScene(*meta["grid"], *meta["frame"])
+    o.set_params(params)
+    o.set_emitters([tuple(e) for e in meta["emitters"]])
+    o.reset()
+    o.step(meta["steps"])
+    for f in firesim.FIELD_NAMES:
+        assert np.array_equal(o.read(f), data[f"field_{f}"]), f
+    assert np.array_equal(o.render_hdr(), data["hdr"])
+    m = o.metrics()
+    for k, v in meta["metrics"].items():
+        assert m[k] == pytest.approx(v, rel=1e-12, abs=1e-300), k
+    o.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_hip_path_matches_fixture(hip_lib, name):
+    data, meta, params = load_case(name)
+    s = firesim.Scene(*meta["grid"], *meta["frame"])
+    s.set_params(params)
+    s.set_emitters([tuple(e) for e in meta["emitters"]])
+    s.reset()
+    s.step(meta["steps"])
+    errs = {f: compare.rel_l2(s.read(f), data[f"field_{f}"]) for f in compare.STEP_FIELDS}
+    print(name, {k: float(f"{v:.2g}") for k, v in errs.items()}, flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= 1e-5}
+    assert not bad, bad
+    assert compare.rel_l2(s.render_hdr(), data["hdr"]) <= 1e-3
+    m = s.metrics()
+    assert m["heat_release_total"] == pytest.approx(meta["metrics"]["heat_release_total"], rel=1e-5)
+    assert m["invalid_cells"] == 0
+    s.close()
