@@ -143,6 +143,7 @@ typedef struct fs_params {
     float sor_omega;             /* [0.05, 1.95]; reference constant 1.52 (:105) */
     int32_t sor_iterations;      /* [1, 1000]; reference constant 40 (:104) */
     uint32_t noise_seed;         /* seed of the integer-hash noise */
+    int32_t ember_count;         /* [0, 192] sparks splatted into the HDR frame (:2323-2452) */
 } fs_params;
 
 /* Spherical fuel + heat source in voxel units (cell i has centre i + 0.5).
@@ -274,6 +275,11 @@ int fs_render_bgra8(fs_scene* scene, uint32_t* bgra_host);   /* frame_w*frame_h 
  * fs_synchronize (or an event after this call) has passed. bgra_host should be
  * pinned (hipHostMalloc) for the copy to overlap other work. */
 int fs_render_bgra8_async(fs_scene* scene, uint32_t* bgra_host);
+/* The HDR frame as FP16 RGBA (4 halves per pixel: max(0, r|g|b) rounded to
+ * nearest even, alpha 1.0), the surface format the reference writes instead
+ * of the tonemap when it renders into a D3D11 texture (packFp16SurfaceKernel
+ * src/fire_cuda.cu:2299-2321, dispatched at :3384-3403). */
+int fs_render_rgba16f(fs_scene* scene, uint16_t* rgba16_host);   /* frame_w*frame_h*4 halves */
 /* Tonemaps a caller HDR frame with the scene's exposure (K21 alone). */
 int fs_tonemap(fs_scene* scene, const float* rgba_host, uint32_t* bgra_host);
 

@@ -61,6 +61,7 @@ fs_params to_params(const FireSettings& s) {
     p.camera_distance = clampf(s.cameraDistance, 1.5f, 6.0f);
     p.cinematic_mode = s.cinematicMode != 0 ? 1 : 0;
     p.raymarch_steps = clampi(s.raymarchSteps, 24, 120);
+    p.ember_count = clampi(s.emberCount, 0, 192);
     p.render_debug_mode = clampi(s.renderDebugMode, 0, 6);
     p.exposure = clampf(s.exposure, 0.45f, 2.40f);
     p.reflection_gain = clampf(s.reflectionGain, 0.0f, 2.2f);

@@ -14,6 +14,7 @@ namespace fs {
 enum ScalarId { HEAT = 0, FUEL, OXYGEN, SOOT, CHARF, ASH, PYRO, PROGRESS, TURB, SOPT, NSCALAR };
 
 constexpr int kMaxEmitters = 16;
+constexpr int kMaxEmbers = 192;  // kMaxEmberCount, src/fire_cuda.cu:110
 constexpr float kAdvectionScale = 0.62f;  // src/fire_cuda.cu:107
 
 // Per-pass parameter block (SimParams src/fire_cuda.cu:16-68), passed by value.
@@ -32,7 +33,8 @@ struct DevParams {
     float burnerX[4], burnerY[4], burnerZ[4];
     float omega;
     int iterations;
-    uint32_t saltNoise2, saltNoise3, saltJitter, saltDither, saltGrain;
+    uint32_t saltNoise2, saltNoise3, saltJitter, saltDither, saltGrain, saltEmber;
+    int emberCount;
     float camEye[3], camForward[3], camRight[3], camUp[3];
     float tanHalfFov, aspect;
     int emitterCount;

@@ -14,6 +14,15 @@ namespace fs {
 constexpr int kMetricSlots = 32;
 constexpr int kMetricBlocks = 1024;
 
+// One spark of the ember pass, as the splat kernel needs it (screen-space
+// centre, core radius, streak radius and direction, life fade, colour and
+// the pixel bounding box; active == 0 for sparks that do not draw).
+struct Ember {
+    float sx, sy, radius, streak, tdx, tdy, fade, r, g, b;
+    int x0, x1, y0, y1, active, pad;
+};
+static_assert(sizeof(Ember) == 64, "one spark is 64 bytes");
+
 struct DeviceScene {
     int nx = 0, ny = 0, nz = 0, lnx = 0, lny = 0, lnz = 0, fw = 0, fh = 0;
     bool bf16 = false;
@@ -28,11 +37,13 @@ struct DeviceScene {
     float* shadow = nullptr;
     float4* hdr = nullptr;
     uint32_t* bgra = nullptr;
+    uint16_t* rgba16 = nullptr;    // FP16 RGBA frame, 4 halves per pixel
     double* partials = nullptr;  // per-block partial sums
     size_t partialCapacity = 0;  // doubles
     double* acc = nullptr;       // [0] heat release of the last step, [1] since reset
     double* metricsOut = nullptr;  // kMetricSlots doubles
     float* clock = nullptr;        // [simulation time, render time]
+    Ember* embers = nullptr;       // kMaxEmbers sparks of the last render
     int sorKernel = 2;           // enum fs_sor_kernel
 };
 
@@ -142,6 +153,7 @@ hipError_t launch_project(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_light(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_render(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_tonemap(DeviceScene& s, const DevParams& p, hipStream_t st);
+hipError_t launch_pack_fp16(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_metrics(DeviceScene& s, const DevParams& p, hipStream_t st);
 // Converts between caller FP32 data and a scalar field's storage type.
 hipError_t launch_pack_scalar(const DeviceScene& s, int field, const float* src, hipStream_t st);

@@ -604,6 +604,170 @@ __global__ __launch_bounds__(256) void k_tonemap(uint32_t* __restrict__ out, con
     out[i] = 0xff000000u | (rr << 16) | (gg << 8) | bb;
 }
 
+// ------------------------------------------------------------- embers ----
+// emberHdrKernel (src/fire_cuda.cu:2323-2452) splats sparks into the HDR frame
+// with float atomicAdd, whose summation order is unspecified. Here it is two
+// passes: k_ember_prepare computes every spark (one thread each), then
+// k_ember_splat has each pixel add the sparks that cover it in spark order, so
+// the frame is deterministic and needs no atomics. The reference's sin-hash
+// per spark (hash21(i, {3.7, 9.1, 19.4, 31.6})) is the integer hash of (i, k).
+__device__ __forceinline__ F3 project_point(const Cam& cam, F3 point) {
+    const F3 view = sub3(point, cam.eye);
+    const float depth = dot3(view, cam.fwd);
+    if (depth <= 0.02f) return f3(-10.0f, -10.0f, -1.0f);
+    const float px = dot3(view, cam.right) / (depth * cam.tanHalfFov * cam.aspect);
+    const float py = dot3(view, cam.up) / (depth * cam.tanHalfFov);
+    return f3(0.5f + px * 0.5f, 0.5f - py * 0.5f, depth);
+}
+
+template <bool B>
+__global__ __launch_bounds__(kMaxEmbers) void k_ember_prepare(Ember* __restrict__ out, Scalars<B> S, const float* __restrict__ U,
+                                                             const float* __restrict__ V, const float* __restrict__ W, DevParams p) {
+    const int i = threadIdx.x;
+    Ember e{};
+    e.active = 0;
+    if (i >= kMaxEmbers) return;
+    const int count = min(max(p.emberCount, 0), kMaxEmbers);
+    const float time = p.clock[1];
+    const int sid = p.sceneId;
+    const float baseGlow = sat(sample_scalar<HEAT, B>(S, p, 0.50f, 0.06f, 0.50f) * 0.42f);
+    const float sparkGate = sstep(0.02f, 0.30f, baseGlow);
+    const float h0 = hash2(i, 0, p.saltEmber), h1 = hash2(i, 1, p.saltEmber), h2 = hash2(i, 2, p.saltEmber), h3 = hash2(i, 3, p.saltEmber);
+    bool live = i < count && p.debugMode == 0 && sparkGate > 0.0001f && !(sid == 1 && h2 > 0.34f) && !(sid == 2 && h0 > 0.14f);
+    if (live) {
+        const float lt = time * (0.20f + h2 * 0.18f + (sid == 2 ? 0.36f : 0.0f)) + h0;
+        const float life = lt - floorf(lt);
+        const float angle = h0 * 6.2831853f;
+        float startX = 0.0f, startZ = 0.0f;
+        if (sid == 2) {
+            const float radius = fmaxf(0.04f, p.emitterRadius);
+            const float cx = p.burnerCount > 0 ? p.burnerX[0] : p.emitterCenterX;
+            const float cz = p.burnerCount > 0 ? p.burnerZ[0] : p.emitterCenterZ;
+            startX = cx + cosf(angle) * radius * (0.82f + h1 * 0.18f);
+            startZ = cz + sinf(angle) * radius * (0.82f + h1 * 0.18f);
+        } else if (sid == 1) {
+            const float logA = capsule_mask(cosf(angle) * 0.34f, sinf(angle) * 0.25f, -0.06f, 0.01f, 0.38f, 0.34f, 0.095f);
+            const float logB = capsule_mask(cosf(angle + 1.7f) * 0.35f, sinf(angle + 1.7f) * 0.25f, 0.07f, -0.02f, -0.58f, 0.33f, 0.092f);
+            const float contact = sat(logA + logB + h3 * 0.35f);
+            const float radius = lerpf(0.16f, 0.46f, sqrtf(h1));
+            startX = cosf(angle) * radius * (0.64f + contact * 0.36f);
+            startZ = sinf(angle) * radius * (0.46f + contact * 0.26f);
+        } else {
+            const float ring = sqrtf(h1);
+            startX = cosf(angle) * 0.48f * ring;
+            startZ = sinf(angle) * 0.48f * ring;
+        }
+        const float su = sat(startX / 2.10f + 0.50f), sw = sat(startZ / 1.64f + 0.50f);
+        const float sv = sid == 2 ? p.emitterHeightNorm : (sid == 1 ? 0.035f + h2 * 0.055f : 0.045f);
+        const Tri tc = tri_scalar_setup<float>(p, su, sv, sw);
+        const float heat = tri_scalar<HEAT, B>(S, tc), charMass = tri_scalar<CHARF, B>(S, tc);
+        const float pyro = tri_scalar<PYRO, B>(S, tc), turb = tri_scalar<TURB, B>(S, tc);
+        const F3 vel = sample_vel<float>(U, V, W, p, su, sv, sw);
+        const float materialGate = sstep(0.018f, 0.40f, charMass + pyro * 0.34f + heat * 0.075f);
+        if (materialGate > 0.0001f) {
+            const float tempK = 620.0f + heat * 330.0f + pyro * 210.0f + charMass * 42.0f;
+            const float lift = sid == 1 ? 0.86f : (sid == 2 ? 0.16f : 0.80f);
+            const float t = life * (sid == 1 ? 1.24f : (sid == 2 ? 0.34f : 1.10f));
+            const float dragK = sid == 2 ? 4.8f : 1.35f;
+            const float rise = 0.26f + h3 * 0.62f + turb * 0.11f;
+            const float fall = 0.22f + h2 * 0.16f;
+            const float driftX = vel.x * 0.34f + (h1 - 0.5f) * 0.20f + p.windX * 0.26f;
+            const float driftZ = vel.z * 0.28f + (h2 - 0.5f) * 0.18f;
+            const float drag = expf(-t * dragK);
+            const F3 world = f3(startX + driftX * t * drag, sv * 2.03f + 0.02f + rise * t * lift - fall * t * t, startZ + driftZ * t * drag);
+            const float pt = fmaxf(0.0f, t - 0.045f);
+            const float pdrag = expf(-pt * dragK);
+            const F3 prevWorld = f3(startX + driftX * pt * pdrag, sv * 2.03f + 0.02f + rise * pt * lift - fall * pt * pt, startZ + driftZ * pt * pdrag);
+            const Cam cam = cam_of(p);
+            const F3 sc = project_point(cam, world), ps = project_point(cam, prevWorld);
+            if (!(sc.z <= 0.0f || sc.x < -0.05f || sc.x > 1.05f || sc.y < -0.05f || sc.y > 1.05f)) {
+                const float size = sstep(720.0f, 1580.0f, tempK) * materialGate;
+                const float radius = ((sid == 2 ? 0.00042f : 0.00086f) + h3 * (sid == 2 ? 0.00042f : 0.00145f)) * (0.72f + size * 0.90f) / fmaxf(0.55f, sc.z);
+                const float cxp = sc.x * static_cast<float>(p.fw) - 0.5f, cyp = sc.y * static_cast<float>(p.fh) - 0.5f;
+                const float trx = sc.x - ps.x, try_ = sc.y - ps.y;
+                const float trailLen = sqrtf(trx * trx + try_ * try_);
+                const float capped = fminf(trailLen, sid == 1 ? 0.016f : 0.005f);
+                const bool hasDir = trailLen > 0.000001f;
+                const int pad = max(2, static_cast<int>(ceilf((radius + capped) * static_cast<float>(max(p.fw, p.fh)) * 3.0f)));
+                const F3 hot = sid == 2 ? f3(0.36f, 0.58f, 1.04f) : f3(1.0f, 0.42f, 0.090f);
+                const F3 cool = sid == 2 ? f3(0.055f, 0.10f, 0.18f) : f3(0.18f, 0.026f, 0.008f);
+                const float cooling = sstep(0.16f, 0.92f, life) * (1.0f - sat(heat * 0.045f));
+                const F3 color = lerp3(hot, cool, cooling);
+                const float fade = sstep(1.0f, 0.08f, life) * sstep(0.0f, 0.16f, life) * sparkGate * materialGate * size *
+                                   (sid == 2 ? 0.055f : (sid == 1 ? 0.58f : 0.82f));
+                e.sx = sc.x;
+                e.sy = sc.y;
+                e.radius = radius;
+                e.streak = radius * (1.0f + capped * 130.0f);
+                e.tdx = hasDir ? trx * (1.0f / trailLen) : 0.0f;
+                e.tdy = hasDir ? try_ * (1.0f / trailLen) : -1.0f;
+                e.fade = fade;
+                e.r = color.x;
+                e.g = color.y;
+                e.b = color.z;
+                e.x0 = max(0, static_cast<int>(floorf(cxp)) - pad);
+                e.x1 = min(p.fw - 1, static_cast<int>(ceilf(cxp)) + pad);
+                e.y0 = max(0, static_cast<int>(floorf(cyp)) - pad);
+                e.y1 = min(p.fh - 1, static_cast<int>(ceilf(cyp)) + pad);
+                e.active = 1;
+            }
+        }
+    }
+    out[i] = e;
+}
+
+__global__ __launch_bounds__(256) void k_ember_splat(float4* __restrict__ hdr, const Ember* __restrict__ E, DevParams p) {
+    __shared__ Ember se[kMaxEmbers];
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+    for (int i = tid; i < kMaxEmbers; i += blockDim.x * blockDim.y) se[i] = E[i];
+    __syncthreads();
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x >= p.fw || y >= p.fh) return;
+    const int count = min(max(p.emberCount, 0), kMaxEmbers);
+    const float ux = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.fw);
+    const float uy = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.fh);
+    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    bool any = false;
+    for (int i = 0; i < count; ++i) {
+        const Ember& e = se[i];
+        if (!e.active || x < e.x0 || x > e.x1 || y < e.y0 || y > e.y1) continue;
+        const float dx = ux - e.sx, dy = uy - e.sy;
+        const float along = dx * e.tdx + dy * e.tdy;
+        const float ax = dx - e.tdx * along, ay = dy - e.tdy * along;
+        const float across2 = ax * ax + ay * ay;
+        const float core = expf(-(across2 / fmaxf(0.0000002f, e.radius * e.radius) + along * along / fmaxf(0.0000002f, e.streak * e.streak)));
+        const float halo = expf(-(dx * dx + dy * dy) / fmaxf(0.0000002f, e.radius * e.radius * 8.0f)) * 0.16f;
+        const float spark = (core + halo) * e.fade;
+        if (spark > 0.00001f) {
+            ar += e.r * spark;
+            ag += e.g * spark;
+            ab += e.b * spark;
+            any = true;
+        }
+    }
+    if (any) {
+        float4 h = hdr[static_cast<size_t>(y) * p.fw + x];
+        h.x += ar;
+        h.y += ag;
+        h.z += ab;
+        hdr[static_cast<size_t>(y) * p.fw + x] = h;
+    }
+}
+
+// ---------------------------------------------------------- FP16 pack ----
+// packFp16SurfaceKernel src/fire_cuda.cu:2299-2321: RGB clamped below at 0 and
+// rounded to nearest-even half, alpha 1.0; one 8-byte store per pixel.
+__device__ __forceinline__ uint32_t half_bits(float v) {
+    return static_cast<uint32_t>(__builtin_bit_cast(uint16_t, static_cast<_Float16>(fmaxf(0.0f, v))));
+}
+__global__ __launch_bounds__(256) void k_pack_fp16(uint2* __restrict__ out, const float4* __restrict__ hdr, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 c = hdr[i];
+    out[i] = make_uint2(half_bits(c.x) | (half_bits(c.y) << 16), half_bits(c.z) | (0x3C00u << 16));
+}
+
 // ========================================================== launchers ====
 template <bool B>
 static Scalars<B> scal(void* const* f) {
@@ -663,6 +827,17 @@ hipError_t launch_render(DeviceScene& s, const DevParams& p, hipStream_t st) {
     const dim3 g((s.fw + 15) / 16, (s.fh + 15) / 16, 1);
     if (s.bf16) k_render<true><<<g, b, 0, st>>>(s.hdr, scal<true>(s.s), s.light, s.shadow, s.u, s.v, s.w, p);
     else k_render<false><<<g, b, 0, st>>>(s.hdr, scal<false>(s.s), s.light, s.shadow, s.u, s.v, s.w, p);
+    if (p.emberCount > 0 && p.debugMode == 0) {
+        if (s.bf16) k_ember_prepare<true><<<1, kMaxEmbers, 0, st>>>(s.embers, scal<true>(s.s), s.u, s.v, s.w, p);
+        else k_ember_prepare<false><<<1, kMaxEmbers, 0, st>>>(s.embers, scal<false>(s.s), s.u, s.v, s.w, p);
+        k_ember_splat<<<g, b, 0, st>>>(s.hdr, s.embers, p);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_fp16(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    const int n = s.fw * s.fh;
+    k_pack_fp16<<<(n + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint2*>(s.rgba16), s.hdr, n);
     return hipGetLastError();
 }
 

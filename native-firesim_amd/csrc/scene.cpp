@@ -67,6 +67,7 @@ void clamp_params(const fs_params& in, fs_params& o) {
     }
     o.sor_omega = clampf(in.sor_omega, 0.05f, 1.95f);
     o.sor_iterations = clampi(in.sor_iterations, 1, 1000);
+    o.ember_count = clampi(in.ember_count, 0, fs::kMaxEmbers);
 }
 
 uint32_t pcg_host(uint32_t v) {
@@ -108,10 +109,12 @@ std::vector<size_t> buffer_sizes(const Dims& d) {
     b.push_back(d.lcells() * 4);
     b.push_back(d.pixels() * 16);
     b.push_back(d.pixels() * 4);
+    b.push_back(d.pixels() * 8);
     b.push_back(d.partials() * 8);
     b.push_back(2 * 8);
     b.push_back(fs::kMetricSlots * 8);
     b.push_back(2 * 4);  // device clock
+    b.push_back(fs::kMaxEmbers * sizeof(fs::Ember));
     return b;
 }
 
@@ -222,6 +225,8 @@ DevParams dev_params(const fs_scene& sc) {
     p.saltJitter = pcg_host(s.noise_seed + 3u * 0x9E3779B9u);
     p.saltDither = pcg_host(s.noise_seed + 4u * 0x9E3779B9u);
     p.saltGrain = pcg_host(s.noise_seed + 5u * 0x9E3779B9u);
+    p.saltEmber = pcg_host(s.noise_seed + 6u * 0x9E3779B9u);
+    p.emberCount = s.ember_count;
     camera_of(s, d.fw, d.fh, p);
     p.emitterCount = static_cast<int>(std::min<size_t>(sc.emitters.size(), fs::kMaxEmitters));
     for (int i = 0; i < p.emitterCount; ++i) {
@@ -395,6 +400,7 @@ void fs_default_params(fs_params* o) {
     o->camera_pitch = 0.18f;
     o->camera_distance = 3.35f;
     o->raymarch_steps = 56;
+    o->ember_count = 96;
     o->exposure = 1.0f;
     o->reflection_gain = 1.0f;
     o->smoke_darkness = 1.0f;
@@ -495,11 +501,13 @@ int create_scene(const fs_scene_desc* desc, void* memory, uint64_t memoryBytes, 
     ds.shadow = static_cast<float*>(take());
     ds.hdr = static_cast<float4*>(take());
     ds.bgra = static_cast<uint32_t*>(take());
+    ds.rgba16 = static_cast<uint16_t*>(take());
     ds.partials = static_cast<double*>(take());
     ds.partialCapacity = s->dims.partials();
     ds.acc = static_cast<double*>(take());
     ds.metricsOut = static_cast<double*>(take());
     ds.clock = static_cast<float*>(take());
+    ds.embers = static_cast<fs::Ember*>(take());
     fs_default_params(&s->prm);
     const int st = reset ? fs_reset(s) : FS_OK;
     if (st != FS_OK) {
@@ -813,6 +821,19 @@ int fs_render_bgra8(fs_scene* s, uint32_t* bgra) {
     if (e == hipSuccess) e = fs::launch_tonemap(s->ds, p, s->stream);
     if (e != hipSuccess) return s->hip_fail("render launch", e);
     e = hipMemcpyAsync(bgra, s->ds.bgra, s->dims.pixels() * 4, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return e == hipSuccess ? FS_OK : s->hip_fail("frame readback", e);
+}
+
+int fs_render_rgba16f(fs_scene* s, uint16_t* rgba16) {
+    if (!s) return global_fail(FS_ERR_NOT_INITIALIZED, "null scene");
+    if (!rgba16) return s->fail(FS_ERR_INVALID_ARGUMENT, "output pixel pointer was null");
+    if (int st = set_device(s)) return st;
+    const DevParams p = dev_params(*s);
+    hipError_t e = fs::launch_render(s->ds, p, s->stream);
+    if (e == hipSuccess) e = fs::launch_pack_fp16(s->ds, p, s->stream);
+    if (e != hipSuccess) return s->hip_fail("render launch", e);
+    e = hipMemcpyAsync(rgba16, s->ds.rgba16, s->dims.pixels() * 8, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     return e == hipSuccess ? FS_OK : s->hip_fail("frame readback", e);
 }

@@ -146,6 +146,7 @@ def settings_to_params(s: FireSettings) -> firesim.Params:
     p.camera_distance = c(s.cameraDistance, 1.5, 6.0)
     p.cinematic_mode = 1 if s.cinematicMode else 0
     p.raymarch_steps = c(s.raymarchSteps, 24, 120)
+    p.ember_count = c(s.emberCount, 0, 192)
     p.render_debug_mode = c(s.renderDebugMode, 0, 6)
     p.exposure, p.reflection_gain = c(s.exposure, 0.45, 2.4), c(s.reflectionGain, 0.0, 2.2)
     p.smoke_darkness = c(s.smokeDarkness, 0.35, 2.4)

@@ -8,7 +8,8 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, Structure, byref, c_char_p, c_double, c_float, c_int32, c_int64, c_uint32, c_uint64, c_void_p
+from ctypes import (POINTER, Structure, byref, c_char_p, c_double, c_float, c_int32, c_int64, c_uint16, c_uint32, c_uint64,
+                    c_void_p)
 
 import numpy as np
 
@@ -60,6 +61,7 @@ class Params(Structure):
         ("burner_center_x", c_float * 4), ("burner_center_y", c_float * 4), ("burner_center_z", c_float * 4),
         ("sor_omega", c_float), ("sor_iterations", c_int32),
         ("noise_seed", c_uint32),
+        ("ember_count", c_int32),
     ]
 
 
@@ -168,6 +170,7 @@ def default_params(**overrides) -> Params:
     p.burner_center_count = 0
     p.sor_omega, p.sor_iterations = 1.52, 40
     p.noise_seed = 0
+    p.ember_count = 96
     for k, v in overrides.items():
         if k in ("burner_center_x", "burner_center_y", "burner_center_z"):
             arr = getattr(p, k)
@@ -257,6 +260,7 @@ def load_library(path: str = LIB_PATH):
         "fs_render_hdr": (c_int32, [c_void_p, P(c_float)]),
         "fs_render_bgra8": (c_int32, [c_void_p, P(c_uint32)]),
         "fs_tonemap": (c_int32, [c_void_p, P(c_float), P(c_uint32)]),
+        "fs_render_rgba16f": (c_int32, [c_void_p, P(c_uint16)]),
         "fs_field_count": (c_int32, [c_void_p, c_int32, P(c_uint64)]),
         "fs_read_field": (c_int32, [c_void_p, c_int32, P(c_float)]),
         "fs_write_field": (c_int32, [c_void_p, c_int32, P(c_float)]),
@@ -460,6 +464,12 @@ class Scene:
         out = np.empty((self.frame_h, self.frame_w), dtype=np.uint32)
         self._check(self.lib.fs_render_bgra8(self.h, out.ctypes.data_as(POINTER(c_uint32))), "fs_render_bgra8")
         return out
+
+    def render_rgba16f(self) -> np.ndarray:
+        """The HDR frame packed as FP16 RGBA (alpha 1), as a float16 array."""
+        out = np.empty((self.frame_h, self.frame_w, 4), dtype=np.uint16)
+        self._check(self.lib.fs_render_rgba16f(self.h, out.ctypes.data_as(POINTER(c_uint16))), "fs_render_rgba16f")
+        return out.view(np.float16)
 
     def tonemap(self, hdr: np.ndarray) -> np.ndarray:
         src = _f32(hdr).reshape(self.frame_h, self.frame_w, 4)

@@ -93,7 +93,7 @@ static inline uint32_t pcg(uint32_t v) {
     return (word >> 22u) ^ word;
 }
 static inline uint32_t salt_of(uint32_t seed, uint32_t k) { return pcg(seed + k * 0x9E3779B9u); }
-enum { kSaltNoise2 = 1, kSaltNoise3 = 2, kSaltJitter = 3, kSaltDither = 4, kSaltGrain = 5 };
+enum { kSaltNoise2 = 1, kSaltNoise3 = 2, kSaltJitter = 3, kSaltDither = 4, kSaltGrain = 5, kSaltEmber = 6 };
 
 static inline R unit24(uint32_t h) { return static_cast<R>(h >> 8) * (1.0 / 16777216.0); }
 static inline R hash2(int32_t x, int32_t y, uint32_t salt) {
@@ -192,6 +192,7 @@ static void clamp_params(const fs_params& in, fs_params& o) {
     o.camera_distance = clampf(in.camera_distance, 1.5f, 6.0f);
     o.cinematic_mode = in.cinematic_mode != 0 ? 1 : 0;
     o.raymarch_steps = clampi(in.raymarch_steps, 24, 256);
+    o.ember_count = clampi(in.ember_count, 0, 192);
     o.render_debug_mode = clampi(in.render_debug_mode, 0, 6);
     o.exposure = clampf(in.exposure, 0.45f, 2.40f);
     o.reflection_gain = clampf(in.reflection_gain, 0.0f, 2.2f);
@@ -233,7 +234,8 @@ struct P {
     R burnerX[4], burnerY[4], burnerZ[4];
     R omega;
     int iterations;
-    uint32_t saltNoise2, saltNoise3, saltJitter, saltDither, saltGrain;
+    uint32_t saltNoise2, saltNoise3, saltJitter, saltDither, saltGrain, saltEmber;
+    int emberCount;
     Cam cam;
     int emitterCount;
     R ex[FS_MAX_EMITTERS], ey[FS_MAX_EMITTERS], ez[FS_MAX_EMITTERS], er[FS_MAX_EMITTERS];
@@ -320,6 +322,8 @@ static P make_p(const State& st, R time) {
     p.saltJitter = salt_of(s.noise_seed, kSaltJitter);
     p.saltDither = salt_of(s.noise_seed, kSaltDither);
     p.saltGrain = salt_of(s.noise_seed, kSaltGrain);
+    p.saltEmber = salt_of(s.noise_seed, kSaltEmber);
+    p.emberCount = s.ember_count;
     p.cam = make_camera(s, st.fw, st.fh);
     p.emitterCount = static_cast<int>(std::min<size_t>(st.emitters.size(), FS_MAX_EMITTERS));
     for (int i = 0; i < p.emitterCount; ++i) {
@@ -1559,6 +1563,111 @@ static V4 render_pixel(const State& st, const P& p, int x, int y) {
     return V4{std::max<R>(0.0, color.x), std::max<R>(0.0, color.y), std::max<R>(0.0, color.z), 1.0};
 }
 
+// emberHdrKernel src/fire_cuda.cu:2323-2452: sparks splatted additively into
+// the HDR frame, spark after spark. The per-spark hashes are hash2(i, k) with
+// k = 0..3 under the ember salt (the reference hashes (i, 3.7 / 9.1 / 19.4 /
+// 31.6) with its sin hash); everything downstream of them is the reference's.
+static void splat_embers(const State& st, const P& p, R* out) {
+    const int count = std::min(std::max(p.emberCount, 0), 192);
+    if (count == 0 || p.debugMode != 0) return;
+    const Cam& cam = p.cam;
+    const R baseGlow = sat(sample_scalar(st, st.s[HEAT].data(), 0.50f, 0.06f, 0.50f) * 0.42f);
+    const R sparkGate = smoothstep(0.02f, 0.30f, baseGlow);
+    if (sparkGate <= 0.0001f) return;
+    const int sid = p.sceneId;
+    for (int i = 0; i < count; ++i) {
+        const R h0 = hash2(i, 0, p.saltEmber), h1 = hash2(i, 1, p.saltEmber);
+        const R h2 = hash2(i, 2, p.saltEmber), h3 = hash2(i, 3, p.saltEmber);
+        if (sid == 1 && h2 > 0.34f) continue;
+        if (sid == 2 && h0 > 0.14f) continue;
+        const R lt = p.time * (0.20f + h2 * 0.18f + (sid == 2 ? 0.36f : 0.0f)) + h0;
+        const R life = lt - std::floor(lt);
+        const R angle = h0 * 6.2831853f;
+        R startX = 0.0, startZ = 0.0;
+        if (sid == 2) {
+            const R radius = std::max<R>(0.04f, p.emitterRadius);
+            const R cx = p.burnerCount > 0 ? p.burnerX[0] : p.emitterCenterX;
+            const R cz = p.burnerCount > 0 ? p.burnerZ[0] : p.emitterCenterZ;
+            startX = cx + std::cos(angle) * radius * (0.82f + h1 * 0.18f);
+            startZ = cz + std::sin(angle) * radius * (0.82f + h1 * 0.18f);
+        } else if (sid == 1) {
+            const R logA = capsule_mask(std::cos(angle) * 0.34f, std::sin(angle) * 0.25f, -0.06f, 0.01f, 0.38f, 0.34f, 0.095f);
+            const R logB =
+                capsule_mask(std::cos(angle + 1.7f) * 0.35f, std::sin(angle + 1.7f) * 0.25f, 0.07f, -0.02f, -0.58f, 0.33f, 0.092f);
+            const R contact = sat(logA + logB + h3 * 0.35f);
+            const R radius = lerp(0.16f, 0.46f, std::sqrt(h1));
+            startX = std::cos(angle) * radius * (0.64f + contact * 0.36f);
+            startZ = std::sin(angle) * radius * (0.46f + contact * 0.26f);
+        } else {
+            const R ring = std::sqrt(h1);
+            startX = std::cos(angle) * 0.48f * ring;
+            startZ = std::sin(angle) * 0.48f * ring;
+        }
+        const R su = sat(startX / 2.10f + 0.50f), sw = sat(startZ / 1.64f + 0.50f);
+        const R sv = sid == 2 ? p.emitterHeightNorm : (sid == 1 ? 0.035f + h2 * 0.055f : 0.045f);
+        const R heat = sample_scalar(st, st.s[HEAT].data(), su, sv, sw);
+        const R charMass = sample_scalar(st, st.s[CHARF].data(), su, sv, sw);
+        const R pyro = sample_scalar(st, st.s[PYRO].data(), su, sv, sw);
+        const R turb = sample_scalar(st, st.s[TURB].data(), su, sv, sw);
+        const V3 vel = sample_velocity(st, st.u.data(), st.v.data(), st.w.data(), su, sv, sw);
+        const R materialGate = smoothstep(0.018f, 0.40f, charMass + pyro * 0.34f + heat * 0.075f);
+        if (materialGate <= 0.0001f) continue;
+        const R tempK = 620.0f + heat * 330.0f + pyro * 210.0f + charMass * 42.0f;
+        const R lift = sid == 1 ? 0.86f : (sid == 2 ? 0.16f : 0.80f);
+        const R t = life * (sid == 1 ? 1.24f : (sid == 2 ? 0.34f : 1.10f));
+        const R dragK = sid == 2 ? 4.8f : 1.35f;
+        const R rise = 0.26f + h3 * 0.62f + turb * 0.11f;
+        const R fall = 0.22f + h2 * 0.16f;
+        const R driftX = vel.x * 0.34f + (h1 - 0.5f) * 0.20f + p.windX * 0.26f;
+        const R driftZ = vel.z * 0.28f + (h2 - 0.5f) * 0.18f;
+        const R drag = std::exp(-t * dragK);
+        const V3 world = v3(startX + driftX * t * drag, sv * 2.03f + 0.02f + rise * t * lift - fall * t * t, startZ + driftZ * t * drag);
+        const R pt = std::max<R>(0.0, t - 0.045f);
+        const R pdrag = std::exp(-pt * dragK);
+        const V3 prev = v3(startX + driftX * pt * pdrag, sv * 2.03f + 0.02f + rise * pt * lift - fall * pt * pt, startZ + driftZ * pt * pdrag);
+        const V3 sc = project_point(cam, world), ps = project_point(cam, prev);
+        if (sc.z <= 0.0 || sc.x < -0.05f || sc.x > 1.05f || sc.y < -0.05f || sc.y > 1.05f) continue;
+        const R size = smoothstep(720.0f, 1580.0f, tempK) * materialGate;
+        const R radius = ((sid == 2 ? 0.00042f : 0.00086f) + h3 * (sid == 2 ? 0.00042f : 0.00145f)) * (0.72f + size * 0.90f) /
+                         std::max<R>(0.55f, sc.z);
+        const R cxp = sc.x * st.fw - 0.5f, cyp = sc.y * st.fh - 0.5f;
+        const R trx = sc.x - ps.x, trY = sc.y - ps.y;
+        const R trailLen = std::sqrt(trx * trx + trY * trY);
+        const R capped = std::min<R>(trailLen, sid == 1 ? 0.016f : 0.005f);
+        const bool hasDir = trailLen > 0.000001f;
+        const R tdx = hasDir ? trx / trailLen : 0.0, tdy = hasDir ? trY / trailLen : -1.0;
+        const int pad = std::max(2, static_cast<int>(std::ceil((radius + capped) * std::max(st.fw, st.fh) * 3.0f)));
+        const int x0 = std::max(0, static_cast<int>(std::floor(cxp)) - pad), x1 = std::min(st.fw - 1, static_cast<int>(std::ceil(cxp)) + pad);
+        const int y0 = std::max(0, static_cast<int>(std::floor(cyp)) - pad), y1 = std::min(st.fh - 1, static_cast<int>(std::ceil(cyp)) + pad);
+        const V3 hot = sid == 2 ? v3(0.36f, 0.58f, 1.04f) : v3(1.0, 0.42f, 0.090f);
+        const V3 cool = sid == 2 ? v3(0.055f, 0.10f, 0.18f) : v3(0.18f, 0.026f, 0.008f);
+        const R cooling = smoothstep(0.16f, 0.92f, life) * (1.0 - sat(heat * 0.045f));
+        const V3 color = lerp3(hot, cool, cooling);
+        const R fade = smoothstep(1.0, 0.08f, life) * smoothstep(0.0, 0.16f, life) * sparkGate * materialGate * size *
+                       (sid == 2 ? 0.055f : (sid == 1 ? 0.58f : 0.82f));
+        const R streak = radius * (1.0 + capped * 130.0f);
+        for (int y = y0; y <= y1; ++y) {
+            const R uy = (y + 0.5) / st.fh;
+            for (int x = x0; x <= x1; ++x) {
+                const R ux = (x + 0.5) / st.fw;
+                const R dx = ux - sc.x, dy = uy - sc.y;
+                const R along = dx * tdx + dy * tdy;
+                const R ax = dx - tdx * along, ay = dy - tdy * along;
+                const R core = std::exp(-((ax * ax + ay * ay) / std::max<R>(0.0000002f, radius * radius) +
+                                          along * along / std::max<R>(0.0000002f, streak * streak)));
+                const R halo = std::exp(-(dx * dx + dy * dy) / std::max<R>(0.0000002f, radius * radius * 8.0f)) * 0.16f;
+                const R spark = (core + halo) * fade;
+                if (spark > 0.00001f) {
+                    R* o = out + 4 * (size_t(y) * st.fw + x);
+                    o[0] += color.x * spark;
+                    o[1] += color.y * spark;
+                    o[2] += color.z * spark;
+                }
+            }
+        }
+    }
+}
+
 static void render_hdr(const State& st, R* out) {
     const P p = make_p(st, st.renderTime);
 #pragma omp parallel for schedule(dynamic, 4)
@@ -1572,6 +1681,7 @@ static void render_hdr(const State& st, R* out) {
             o[3] = c.w;
         }
     }
+    splat_embers(st, p, out);
 }
 
 // ------------------------------------------------------------- tonemap ----

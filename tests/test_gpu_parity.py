@@ -108,6 +108,42 @@ def test_render_parity(hip_lib, warm):
     assert np.all(hs[..., 3] == 1.0)
 
 
+@pytest.mark.parametrize("scene_id", [0, 1, 2, 3])
+def test_ember_parity(hip_lib, scene_id):
+    """The spark increment render(192 sparks) - render(0) against the oracle's (rel L2 <= 1e-3)."""
+    p = compare.emitter_params(scene_id=scene_id, burner_center_count=0, ember_count=192, camera_distance=2.4)
+    em = [(20.0, 3.0, 16.0, 5.0, 1.0, 1500.0)]
+    state, t, rt = compare.warm_state(40, 48, 32, p, em, steps=20)
+    s, o = _pair(p, em, grid=(40, 48, 32), frame=(96, 64))
+    compare.load_both(s, o, state, t, rt)
+    hs1, ho1 = s.render_hdr(), o.render_hdr()
+    p.ember_count = 0
+    s.set_params(p)
+    o.set_params(p)
+    hs0, ho0 = s.render_hdr(), o.render_hdr()
+    inc_s, inc_o = hs1 - hs0.astype(np.float64), ho1 - ho0
+    print(scene_id, "spark pixels", int((inc_o[..., :3].sum(-1) > 0).sum()), flush=True)
+    if scene_id != 2:
+        assert inc_o.max() > 0.0
+    assert compare.rel_l2(inc_s[..., :3], inc_o[..., :3]) <= 1e-3
+    assert compare.rel_l2(hs1[..., :3], ho1[..., :3]) <= 1e-3
+    assert np.all(hs1[..., 3] == 1.0)
+
+
+def test_fp16_pack(hip_lib, warm):
+    """FP16 RGBA frame == numpy's round-to-nearest-even of max(0, HDR), alpha 1, bit for bit."""
+    params, state, t, rt = warm
+    s, o = _pair(params, EMITTERS, frame=(64, 48))
+    compare.load_both(s, o, state, t, rt)
+    hdr = s.render_hdr()
+    packed = s.render_rgba16f()
+    want = np.maximum(hdr, 0.0).astype(np.float16)
+    want[..., 3] = 1.0
+    assert packed.shape == (48, 64, 4)
+    assert np.array_equal(packed.view(np.uint16), want.view(np.uint16))
+    assert hdr[..., :3].max() > 0.0
+
+
 def test_tonemap_parity(hip_lib, warm):
     params, state, t, rt = warm
     s, o = _pair(params, EMITTERS, frame=(64, 48))

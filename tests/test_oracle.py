@@ -25,6 +25,9 @@ function is pinned by a check that a plausible mistake would fail:
   render + tonemap (:1953-2297)                      empty fields -> room only;
         debug mode 3 is the transmittance; dither bounded by 1/2 LSB
   clamps (:3135-3179)                                product fs_clamp_params == oracle
+  embers (:2323-2452)                                additive and non-negative; absent in
+        debug modes; campfire/burner spawn gates on the right hash (k=2 / k=0);
+        each spark is rank-1 in its colour, between the hot and cooled colours
 
 "Parity unpinned" items: none of the above is compared against the reference's
 own sin-hash noise (not portable, replaced by the integer hash by design).
@@ -375,3 +378,73 @@ def test_clamps_match_product():
         assert lib.fs_clamp_params(ctypes.byref(p), ctypes.byref(mine)) == 0
         theirs = oracle.clamp_params(p)
         assert bytes(mine) == bytes(theirs)
+
+
+def _pcg(v: int) -> int:
+    """PCG hash (Jarzynski & Olano 2020), the integer hash of the noise and the sparks."""
+    state = (v * 747796405 + 2891336453) & 0xFFFFFFFF
+    word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) & 0xFFFFFFFF
+    return (word >> 22) ^ word
+
+
+def _ember_increments(scene_id, count):
+    """HDR increment of each spark k: render(k + 1 sparks) - render(k sparks)."""
+    p = compare.emitter_params(scene_id=scene_id, ember_count=0, burner_center_count=0)
+    o = oracle.OracleScene(24, 28, 20, 48, 32)
+    o.set_params(p)
+    o.set_emitters([(12.0, 2.0, 10.0, 4.0, 1.0, 1500.0)])
+    o.reset()
+    o.step(10)
+    prev = o.render_hdr()
+    base = prev
+    incs = []
+    for k in range(1, count + 1):
+        p.ember_count = k
+        o.set_params(p)
+        h = o.render_hdr()
+        incs.append(h - prev)
+        prev = h
+    return o, p, base, prev, incs
+
+
+def test_embers_additive_and_gated():
+    salt = _pcg((7 + 6 * 0x9E3779B9) & 0xFFFFFFFF)  # noise_seed 7, the ember salt
+    for scene_id, gate_k, gate_at in ((1, 2, F32(0.34)), (0, None, None), (3, None, None)):
+        o, p, base, full, incs = _ember_increments(scene_id, 64)
+        assert np.all(full[..., 3] == base[..., 3])
+        lit = 0
+        for k, inc in enumerate(incs):
+            assert inc.min() >= 0.0
+            h = [L.or_hash2(k, j, salt) for j in range(4)]
+            if gate_k is not None and h[gate_k] > gate_at:
+                assert inc.max() == 0.0, (scene_id, k)
+                continue
+            if inc.max() == 0.0:
+                continue
+            lit += 1
+            # one spark is colour x a scalar footprint; the colour lies between the
+            # cooled ember (0.18, 0.026, 0.008) and the hot core (1.0, 0.42, 0.09)
+            rgb = inc[..., :3].reshape(-1, 3)
+            rgb = rgb[rgb[:, 0] > 1e-9]
+            gr, br = rgb[:, 1] / rgb[:, 0], rgb[:, 2] / rgb[:, 0]
+            assert np.ptp(gr) <= 1e-9 and np.ptp(br) <= 1e-9
+            assert F32(0.026) / F32(0.18) - 1e-9 <= gr[0] <= F32(0.42) + 1e-9
+            assert F32(0.008) / F32(0.18) - 1e-9 <= br[0] <= F32(0.09) + 1e-9
+        assert lit >= 2, scene_id
+        p.render_debug_mode = 3
+        p.ember_count = 64
+        o.set_params(p)
+        with_sparks = o.render_hdr()
+        p.ember_count = 0
+        o.set_params(p)
+        assert np.array_equal(with_sparks, o.render_hdr())
+        o.close()
+
+
+def test_burner_embers_gate_on_first_hash():
+    salt = _pcg((7 + 6 * 0x9E3779B9) & 0xFFFFFFFF)
+    o, p, base, full, incs = _ember_increments(2, 48)
+    for k, inc in enumerate(incs):
+        if L.or_hash2(k, 0, salt) > F32(0.14):
+            assert inc.max() == 0.0, k
+    o.close()
