@@ -95,9 +95,19 @@ def cpu_baseline(name: str, seed: int, budget_s: float):
             "sample": f"FP64 oracle, same {cfg['grid'][0]}x{cfg['grid'][1]}x{cfg['grid'][2]} workload from reset: 2 untimed + {n} timed steps, {dt:.1f} s"}
 
 
-def read_traffic(kernel_prefixes, grid):
+def profile_kernel_name(sor_kernel: str) -> str:
+    """rocprofv3's short name of the SOR kernel a scene runs: tb_t24_z16_k4 ->
+    k_sor_tb<24,4,16> (tile edge, half-sweeps per launch, tile depth)."""
+    if sor_kernel == "half_sweep":
+        return "k_sor_half"
+    parts = dict((tok[0], int(tok[1:])) for tok in sor_kernel.split("_")[1:])
+    return f"k_sor_tb<{parts['t']},{parts['k']},{parts.get('z', parts['t'])}>"
+
+
+def read_traffic(kernel_names, grid):
     """HBM bytes per dispatch measured with rocprofv3 PMC counters (FETCH_SIZE,
-    WRITE_SIZE, copy-calibrated), from the committed profile summary."""
+    WRITE_SIZE, copy-calibrated), from the committed profile summary; None when
+    the summary has no line for this exact kernel and grid."""
     path = os.path.join(REPO, "profiles", "pmc_traffic.txt")
     if not os.path.exists(path):
         return None
@@ -106,7 +116,7 @@ def read_traffic(kernel_prefixes, grid):
         if line.startswith("#") or "=" not in line:
             continue
         kv = dict(tok.split("=", 1) for tok in line.split() if "=" in tok)
-        if kv.get("grid") == gtag and any(kv.get("kernel", "").startswith(k) for k in kernel_prefixes):
+        if kv.get("grid") == gtag and kv.get("kernel") in kernel_names:
             return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
     return None
 
@@ -130,6 +140,11 @@ def make_scene(grid, params, emitters, storage=firesim.STORAGE_FP32, frame=(64, 
     s.set_emitters(emitters)
     s.reset()
     return s
+
+
+def progress(msg: str) -> None:
+    """One line on stderr per finished section (stdout carries only the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def pcts(ms) -> dict:
@@ -303,6 +318,7 @@ def main():
     # Measured region: the same number of steps with HIP events recorded on the
     # scene stream between passes (direct launches), for the per-pass times and
     # the dominant kernel's time per launch.
+    progress(f"single scene: {args.steps} timed steps, {value / 1e9:.3f} G voxel-steps/s")
     times, _ = scene.step_timed(args.steps)
     stages = {}
     for i, name in enumerate(firesim.STAGE_NAMES):
@@ -315,10 +331,14 @@ def main():
     launch_ms = stages[dominant]["ms"] / launches
     bytes_per_launch = PASS_BYTES[dominant] * cells * (params.sor_iterations / launches if dominant == "pressure" else 1.0 / launches)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = read_traffic(PASS_KERNELS[dominant], cfg["grid"])
+    if dominant == "pressure":
+        kernel_names = [profile_kernel_name(firesim.SOR_KERNELS[scene.sor_kernel()])]
+    else:
+        kernel_names = [PASS_KERNELS[dominant][0], PASS_KERNELS[dominant][0] + "<false>"]
+    traffic = read_traffic(kernel_names, cfg["grid"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": PASS_KERNELS[dominant][0], "pass": dominant, "launch_ms": round(launch_ms, 5),
+                "kernel": kernel_names[0], "pass": dominant, "launch_ms": round(launch_ms, 5),
                 "alg_bytes_per_launch": bytes_per_launch}
 
     render = None
@@ -332,8 +352,11 @@ def main():
     scene_sor_kernel = scene.sor_kernel()
     scene.close()
 
+    progress("per-pass timing and render done")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.config, args.seed, args.cpu_budget)
+    progress("cpu baseline done")
     batched = None if args.no_batched else bench_batched(args.seed, args.batched_steps, args.batched_scenes)
+    progress("batched done")
     served = None if args.no_served else bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb,
                                                       firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16,
                                                       not args.served_no_graph)

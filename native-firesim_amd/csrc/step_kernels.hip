@@ -420,6 +420,14 @@ __global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, flo
 }
 
 // -------------------------------------------------------------- K12 ----
+// The SOR cell update of sorPressureParityKernel (src/fire_cuda.cu:1201-1227):
+// neighbours summed in the order L, R, D, U, B, F after d. Every SOR kernel
+// calls this one function, so all of them evaluate the same FP32 expression.
+__device__ __forceinline__ float sor_update(float dv, float pc, float pL, float pR, float pD, float pU, float pB, float pF, float omega) {
+    const float projected = (dv + pL + pR + pD + pU + pB + pF) * (1.0f / 6.0f);
+    return pc + omega * (projected - pc);
+}
+
 // One red/black half-sweep, in place: the updated colour (x+y+z)&1 == parity
 // reads only the other colour, so every cell of a half-sweep is independent.
 // Thread (tx, y, z) of a ((nx+1)/2, ny, nz) grid updates cell x = 2tx + colour.
@@ -437,8 +445,7 @@ __device__ __forceinline__ void sor_half_at(float* __restrict__ P, const float* 
     const float pU = y < ny - 1 ? P[idx + row] : pc;
     const float pB = z > 0 ? P[idx - plane] : pc;
     const float pF = z < nz - 1 ? P[idx + plane] : pc;
-    const float projected = (D[idx] + pL + pR + pD + pU + pB + pF) * (1.0f / 6.0f);
-    P[idx] = pc + omega * (projected - pc);
+    P[idx] = sor_update(D[idx], pc, pL, pR, pD, pU, pB, pF, omega);
 }
 
 __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const float* __restrict__ D, int parity, float omega, int nx, int ny, int nz) {
@@ -524,8 +531,7 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
                 const float pU = sp[i + oU];
                 const float pB = sp[i + (gz > 0 ? -NT : 0)];
                 const float pF = sp[i + (gz < nz - 1 ? NT : 0)];
-                const float projected = (dv + pL + pR + pD + pU + pB + pF) * (1.0f / 6.0f);
-                sp[i] = pc + omega * (projected - pc);
+                sp[i] = sor_update(dv, pc, pL, pR, pD, pU, pB, pF, omega);
             }
         }
         __syncthreads();
