@@ -102,7 +102,7 @@ enum fs_scene_kind {
 typedef struct fs_scene fs_scene;
 
 typedef struct fs_scene_desc {
-    int32_t nx, ny, nz;        /* grid cells; each >= 4 */
+    int32_t nx, ny, nz;        /* grid cells; each >= 4, (nx+1)(ny+1)(nz+1) <= 2^30 */
     int32_t frame_w, frame_h;  /* render target; each >= 1 */
     int32_t storage;           /* enum fs_storage */
     int32_t device;            /* HIP device ordinal */

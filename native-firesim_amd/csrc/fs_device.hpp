@@ -45,6 +45,22 @@ struct DevParams {
     const float* clock;
 };
 
+// Element index into a device array. Scene creation rejects grids with more
+// than 2^30 values in any field ((nx+1)(ny+1)(nz+1) <= 2^30), so an element's
+// byte offset fits 32 bits. ld_at/st_at address an element as uniform base +
+// 32-bit byte offset, which the loads take directly (no 64-bit address math).
+using Idx = uint32_t;
+constexpr double kMaxFieldValues = 1073741824.0;  // 2^30
+
+template <class T>
+__device__ __forceinline__ T ld_at(const T* __restrict__ base, Idx i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<Idx>(sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ void st_at(T* __restrict__ base, Idx i, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<Idx>(sizeof(T))) = v;
+}
+
 // Scalar field set. When BF16 is true, heat, fuel and soot are stored as
 // bfloat16 (round to nearest even) and widened to FP32 on load.
 template <bool BF16>
@@ -54,22 +70,22 @@ struct Scalars {
     static constexpr __host__ __device__ bool narrow(int k) { return BF16 && (k == HEAT || k == FUEL || k == SOOT); }
 
     template <int K>
-    __device__ __forceinline__ float ld(size_t i) const {
+    __device__ __forceinline__ float ld(Idx i) const {
         if constexpr (narrow(K)) {
-            const uint32_t b = static_cast<uint32_t>(static_cast<const uint16_t*>(f[K])[i]) << 16;
+            const uint32_t b = static_cast<uint32_t>(ld_at(static_cast<const uint16_t*>(f[K]), i)) << 16;
             return __uint_as_float(b);
         } else {
-            return static_cast<const float*>(f[K])[i];
+            return ld_at(static_cast<const float*>(f[K]), i);
         }
     }
     template <int K>
-    __device__ __forceinline__ void st(size_t i, float v) const {
+    __device__ __forceinline__ void st(Idx i, float v) const {
         if constexpr (narrow(K)) {
             uint32_t b = __float_as_uint(v);
             b += 0x7FFFu + ((b >> 16) & 1u);
-            static_cast<uint16_t*>(f[K])[i] = static_cast<uint16_t>(b >> 16);
+            st_at(static_cast<uint16_t*>(f[K]), i, static_cast<uint16_t>(b >> 16));
         } else {
-            static_cast<float*>(f[K])[i] = v;
+            st_at(static_cast<float*>(f[K]), i, v);
         }
     }
 };
@@ -191,7 +207,7 @@ __device__ inline float fbm3(float px, float py, float pz, uint32_t salt) {
 // Clamped trilinear sample of an x-fastest array of size dx*dy*dz with
 // coordinate multipliers sx, sy, sz (src/fire_cuda.cu:489-600).
 struct Tri {
-    size_t i000, i100, i010, i110, i001, i101, i011, i111;
+    Idx i000, i100, i010, i110, i001, i101, i011, i111;
     float tx, ty, tz;
 };
 
@@ -219,14 +235,19 @@ __device__ __forceinline__ Tri tri_setup(int dx, int dy, int dz, T sx, T sy, T s
     t.tx = static_cast<float>(fx - flx);
     t.ty = static_cast<float>(fy - fly);
     t.tz = static_cast<float>(fz - flz);
-    const size_t r00 = (static_cast<size_t>(z0) * dy + y0) * dx;
-    const size_t r10 = (static_cast<size_t>(z0) * dy + y1) * dx;
-    const size_t r01 = (static_cast<size_t>(z1) * dy + y0) * dx;
-    const size_t r11 = (static_cast<size_t>(z1) * dy + y1) * dx;
-    t.i000 = r00 + x0; t.i100 = r00 + x1;
-    t.i010 = r10 + x0; t.i110 = r10 + x1;
-    t.i001 = r01 + x0; t.i101 = r01 + x1;
-    t.i011 = r11 + x0; t.i111 = r11 + x1;
+    // corner (x0, y0, z0) and the steps to the +1 corners (0 where clamped)
+    const Idx row = static_cast<Idx>(dx), plane = row * static_cast<Idx>(dy);
+    const Idx ox = static_cast<Idx>(x1 - x0);
+    const Idx oy = y1 != y0 ? row : 0u;
+    const Idx oz = z1 != z0 ? plane : 0u;
+    t.i000 = static_cast<Idx>(z0) * plane + static_cast<Idx>(y0) * row + static_cast<Idx>(x0);
+    t.i100 = t.i000 + ox;
+    t.i010 = t.i000 + oy;
+    t.i110 = t.i010 + ox;
+    t.i001 = t.i000 + oz;
+    t.i101 = t.i001 + ox;
+    t.i011 = t.i001 + oy;
+    t.i111 = t.i011 + ox;
     return t;
 }
 
@@ -240,7 +261,8 @@ __device__ __forceinline__ float tri_blend(const Tri& t, float c000, float c100,
 }
 
 __device__ __forceinline__ float tri_plain(const float* __restrict__ f, const Tri& t) {
-    return tri_blend(t, f[t.i000], f[t.i100], f[t.i010], f[t.i110], f[t.i001], f[t.i101], f[t.i011], f[t.i111]);
+    return tri_blend(t, ld_at(f, t.i000), ld_at(f, t.i100), ld_at(f, t.i010), ld_at(f, t.i110), ld_at(f, t.i001), ld_at(f, t.i101),
+                     ld_at(f, t.i011), ld_at(f, t.i111));
 }
 
 template <int K, bool B>
@@ -281,8 +303,8 @@ __device__ __forceinline__ F3 sample_vel(const float* __restrict__ U, const floa
 __device__ __forceinline__ float4 sample_light(const float4* __restrict__ L, const DevParams& p, float u, float v, float w) {
     const Tri t = tri_setup(p.lnx, p.lny, p.lnz, static_cast<float>(p.lnx - 1), static_cast<float>(p.lny - 1),
                             static_cast<float>(p.lnz - 1), u, v, w);
-    const float4 a = L[t.i000], b = L[t.i100], c = L[t.i010], d = L[t.i110];
-    const float4 e = L[t.i001], f = L[t.i101], g = L[t.i011], h = L[t.i111];
+    const float4 a = ld_at(L, t.i000), b = ld_at(L, t.i100), c = ld_at(L, t.i010), d = ld_at(L, t.i110);
+    const float4 e = ld_at(L, t.i001), f = ld_at(L, t.i101), g = ld_at(L, t.i011), h = ld_at(L, t.i111);
     float4 o;
     o.x = tri_blend(t, a.x, b.x, c.x, d.x, e.x, f.x, g.x, h.x);
     o.y = tri_blend(t, a.y, b.y, c.y, d.y, e.y, f.y, g.y, h.y);
@@ -297,7 +319,7 @@ __device__ __forceinline__ float sample_shadow(const float* __restrict__ S, cons
     const int x = min(max(static_cast<int>(u * static_cast<float>(p.lnx - 1) + 0.5f), 0), p.lnx - 1);
     const int y = min(max(static_cast<int>(v * static_cast<float>(p.lny - 1) + 0.5f), 0), p.lny - 1);
     const int z = min(max(static_cast<int>(w * static_cast<float>(p.lnz - 1) + 0.5f), 0), p.lnz - 1);
-    return S[(static_cast<size_t>(z) * p.lny + y) * p.lnx + x];
+    return ld_at(S, (static_cast<Idx>(z) * static_cast<Idx>(p.lny) + static_cast<Idx>(y)) * static_cast<Idx>(p.lnx) + static_cast<Idx>(x));
 }
 
 // --------------------------------------------------------- fuel beds ----

@@ -133,9 +133,11 @@ bool valid_desc(const fs_scene_desc* d, std::string& why) {
         why = "grid dimensions must each be >= 4";
         return false;
     }
-    const double cells = static_cast<double>(d->nx) * d->ny * d->nz;
-    if (cells > 2.0e9) {
-        why = "grid too large";
+    // Kernels index every field with 32-bit byte offsets: no array may hold
+    // more than 2^30 values (the face arrays hold up to (nx+1)(ny+1)(nz+1)).
+    const double values = static_cast<double>(d->nx + 1) * (d->ny + 1) * (d->nz + 1);
+    if (values > fs::kMaxFieldValues) {
+        why = "grid too large: (nx+1)(ny+1)(nz+1) must not exceed 2^30";
         return false;
     }
     if (d->frame_w < 1 || d->frame_h < 1 || static_cast<double>(d->frame_w) * d->frame_h > 1.0e8) {

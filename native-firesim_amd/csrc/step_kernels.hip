@@ -23,17 +23,17 @@ size_t react_partial_count(int nx, int ny, int nz) {
     return static_cast<size_t>(g.x) * g.y * g.z;
 }
 
-__device__ __forceinline__ size_t sidx(const DevParams& p, int x, int y, int z) {
-    return (static_cast<size_t>(z) * p.ny + y) * p.nx + x;
+__device__ __forceinline__ Idx sidx(const DevParams& p, int x, int y, int z) {
+    return (static_cast<Idx>(z) * static_cast<Idx>(p.ny) + static_cast<Idx>(y)) * static_cast<Idx>(p.nx) + static_cast<Idx>(x);
 }
-__device__ __forceinline__ size_t uidx(const DevParams& p, int x, int y, int z) {
-    return (static_cast<size_t>(z) * p.ny + y) * (p.nx + 1) + x;
+__device__ __forceinline__ Idx uidx(const DevParams& p, int x, int y, int z) {
+    return (static_cast<Idx>(z) * static_cast<Idx>(p.ny) + static_cast<Idx>(y)) * static_cast<Idx>(p.nx + 1) + static_cast<Idx>(x);
 }
-__device__ __forceinline__ size_t vidx(const DevParams& p, int x, int y, int z) {
-    return (static_cast<size_t>(z) * (p.ny + 1) + y) * p.nx + x;
+__device__ __forceinline__ Idx vidx(const DevParams& p, int x, int y, int z) {
+    return (static_cast<Idx>(z) * static_cast<Idx>(p.ny + 1) + static_cast<Idx>(y)) * static_cast<Idx>(p.nx) + static_cast<Idx>(x);
 }
-__device__ __forceinline__ size_t widx(const DevParams& p, int x, int y, int z) {
-    return (static_cast<size_t>(z) * p.ny + y) * p.nx + x;
+__device__ __forceinline__ Idx widx(const DevParams& p, int x, int y, int z) {
+    return (static_cast<Idx>(z) * static_cast<Idx>(p.ny) + static_cast<Idx>(y)) * static_cast<Idx>(p.nx) + static_cast<Idx>(x);
 }
 
 // Sum over the 256-thread block in FP64; thread 0 returns the total.
@@ -64,7 +64,7 @@ __device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float
             const F3 vel = sample_vel<double>(U, V, W, p, fu, fv, fw);
             out = sample_u<double>(U, p, fu - vel.x * k, fv - vel.y * k, fw - vel.z * k) * 0.995f;
         }
-        uo[uidx(p, x, y, z)] = out;
+        st_at(uo, uidx(p, x, y, z), out);
     }
     if (x < p.nx && z < p.nz) {
         float out = 0.0f;
@@ -73,7 +73,7 @@ __device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float
             const F3 vel = sample_vel<double>(U, V, W, p, fu, fv, fw);
             out = sample_v<double>(V, p, fu - vel.x * k, fv - vel.y * k, fw - vel.z * k) * 0.996f;
         }
-        vo[vidx(p, x, y, z)] = out;
+        st_at(vo, vidx(p, x, y, z), out);
     }
     if (x < p.nx && y < p.ny) {
         float out = 0.0f;
@@ -82,7 +82,7 @@ __device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float
             const F3 vel = sample_vel<double>(U, V, W, p, fu, fv, fw);
             out = sample_w<double>(W, p, fu - vel.x * k, fv - vel.y * k, fw - vel.z * k) * 0.995f;
         }
-        wo[widx(p, x, y, z)] = out;
+        st_at(wo, widx(p, x, y, z), out);
     }
 }
 
@@ -118,7 +118,7 @@ __device__ __forceinline__ float maccormack(const Scalars<B>& S, const Tri& tp, 
 template <bool B>
 __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const float* __restrict__ U, const float* __restrict__ V,
                             const float* __restrict__ W, const DevParams& p, int x, int y, int z) {
-    const size_t idx = sidx(p, x, y, z);
+    const Idx idx = sidx(p, x, y, z);
     const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
     const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
     const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
@@ -257,9 +257,9 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
     soot *= expf(-dt * (0.060f + v * 0.092f + oxygen * 0.028f + bA * 0.96f + upper * 0.22f));
     pyro *= expf(-dt * (1.55f + v * 0.58f + bA * 0.18f));
 
-    const float uL = U[uidx(p, x, y, z)], uR = U[uidx(p, x + 1, y, z)];
-    const float vD = V[vidx(p, x, y, z)], vU = V[vidx(p, x, y + 1, z)];
-    const float wB = W[widx(p, x, y, z)], wF = W[widx(p, x, y, z + 1)];
+    const float uL = ld_at(U, uidx(p, x, y, z)), uR = ld_at(U, uidx(p, x + 1, y, z));
+    const float vD = ld_at(V, vidx(p, x, y, z)), vU = ld_at(V, vidx(p, x, y + 1, z));
+    const float wB = ld_at(W, widx(p, x, y, z)), wF = ld_at(W, widx(p, x, y, z + 1));
     const float shear = fabsf(uR - uL) + fabsf(vU - vD) + fabsf(wF - wB);
     const float flameActivity = sat(progress * 0.64f + burnMass * 5.2f + heat * 0.065f);
     turb += dt * (shear * (0.72f + progress * 0.06f) + flameActivity * p.turbulence * 0.62f + front * 0.085f + soot * 0.018f);
@@ -318,7 +318,7 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
                                                  const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                  const DevParams& p, int x, int y, int z) {
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
-    const size_t idx = sidx(p, x, y, z);
+    const Idx idx = sidx(p, x, y, z);
     const int sid = p.sceneId;
     const float h = S.template ld<HEAT>(idx), s = S.template ld<SOOT>(idx), k = S.template ld<TURB>(idx), c = S.template ld<PROGRESS>(idx);
     const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
@@ -330,9 +330,9 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
     const float gradX = S.template ld<HEAT>(sidx(p, xp, y, z)) - S.template ld<HEAT>(sidx(p, xm, y, z));
     const float gradY = S.template ld<HEAT>(sidx(p, x, yp, z)) - S.template ld<HEAT>(sidx(p, x, ym, z));
     const float gradZ = S.template ld<HEAT>(sidx(p, x, y, zp)) - S.template ld<HEAT>(sidx(p, x, y, zm));
-    const float uL = U[uidx(p, x, y, z)], uR = U[uidx(p, x + 1, y, z)];
-    const float vD = V[vidx(p, x, y, z)], vU = V[vidx(p, x, y + 1, z)];
-    const float wB = W[widx(p, x, y, z)], wF = W[widx(p, x, y, z + 1)];
+    const float uL = ld_at(U, uidx(p, x, y, z)), uR = ld_at(U, uidx(p, x + 1, y, z));
+    const float vD = ld_at(V, vidx(p, x, y, z)), vU = ld_at(V, vidx(p, x, y + 1, z));
+    const float wB = ld_at(W, widx(p, x, y, z)), wF = ld_at(W, widx(p, x, y, z + 1));
     const float curlX = (wF - wB) - (vU - vD);
     const float curlY = (uR - uL) - (wF - wB);
     const float curlZ = (vU - vD) - (uR - uL);
@@ -351,9 +351,9 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
     const float vertDamp = 0.5f * (vD + vU) * (0.10f + upperDrag * 0.72f + s * 0.020f);
     const float buoyancy = h * buoyScale * (1.62f + p.turbulence * 0.34f + c * 0.16f) * buoyFade - s * 0.095f - gradY * 0.18f - vertDamp;
     const float windProfile = 0.34f + v * 0.96f;
-    cv[idx] = p.dt * (buoyancy + curlY * vortexGain * 0.22f);
-    cu[idx] = p.dt * (p.windX * windProfile + (-gradZ * 1.36f + noiseA * (0.20f + upperDrag * 0.16f) + curlX * vortexGain) * curlGain);
-    cw[idx] = p.dt * (p.windZ * windProfile + (gradX * 1.36f + noiseB * (0.20f + upperDrag * 0.16f) + curlZ * vortexGain) * curlGain);
+    st_at(cv, idx, p.dt * (buoyancy + curlY * vortexGain * 0.22f));
+    st_at(cu, idx, p.dt * (p.windX * windProfile + (-gradZ * 1.36f + noiseA * (0.20f + upperDrag * 0.16f) + curlX * vortexGain) * curlGain));
+    st_at(cw, idx, p.dt * (p.windZ * windProfile + (gradX * 1.36f + noiseB * (0.20f + upperDrag * 0.16f) + curlZ * vortexGain) * curlGain));
 }
 
 template <bool B>
@@ -372,22 +372,22 @@ __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, f
 // FP32 rounding would dominate its relative error.
 __device__ __forceinline__ double face_u(const float* U, const float* cu, const DevParams& p, int fx, int y, int z) {
     if (fx == 0 || fx == p.nx) return 0.0;
-    double val = U[uidx(p, fx, y, z)];
-    val += cu[sidx(p, fx - 1, y, z)];
-    val += cu[sidx(p, fx, y, z)];
+    double val = ld_at(U, uidx(p, fx, y, z));
+    val += ld_at(cu, sidx(p, fx - 1, y, z));
+    val += ld_at(cu, sidx(p, fx, y, z));
     return fmin(4.0, fmax(-4.0, val));
 }
 __device__ __forceinline__ double face_v(const float* V, const float* cv, const DevParams& p, int x, int fy, int z) {
     if (fy == 0) return 0.0;
-    double val = V[vidx(p, x, fy, z)];
-    val += cv[sidx(p, x, fy - 1, z)];
+    double val = ld_at(V, vidx(p, x, fy, z));
+    val += ld_at(cv, sidx(p, x, fy - 1, z));
     return fmin(static_cast<double>(6.4f), fmax(static_cast<double>(-0.9f), val));  // the reference's float bounds
 }
 __device__ __forceinline__ double face_w(const float* W, const float* cw, const DevParams& p, int x, int y, int fz) {
     if (fz == 0 || fz == p.nz) return 0.0;
-    double val = W[widx(p, x, y, fz)];
-    val += cw[sidx(p, x, y, fz - 1)];
-    val += cw[sidx(p, x, y, fz)];
+    double val = ld_at(W, widx(p, x, y, fz));
+    val += ld_at(cw, sidx(p, x, y, fz - 1));
+    val += ld_at(cw, sidx(p, x, y, fz));
     return fmin(4.0, fmax(-4.0, val));
 }
 
@@ -399,16 +399,16 @@ __device__ __forceinline__ void force_apply_at(float* __restrict__ uo, float* __
     const double uL = face_u(U, cu, p, x, y, z), uR = face_u(U, cu, p, x + 1, y, z);
     const double vD = face_v(V, cv, p, x, y, z), vU = face_v(V, cv, p, x, y + 1, z);
     const double wB = face_w(W, cw, p, x, y, z), wF = face_w(W, cw, p, x, y, z + 1);
-    uo[uidx(p, x, y, z)] = static_cast<float>(uL);
-    if (x == p.nx - 1) uo[uidx(p, x + 1, y, z)] = static_cast<float>(uR);
-    vo[vidx(p, x, y, z)] = static_cast<float>(vD);
-    if (y == p.ny - 1) vo[vidx(p, x, y + 1, z)] = static_cast<float>(vU);
-    wo[widx(p, x, y, z)] = static_cast<float>(wB);
-    if (z == p.nz - 1) wo[widx(p, x, y, z + 1)] = static_cast<float>(wF);
+    st_at(uo, uidx(p, x, y, z), static_cast<float>(uL));
+    if (x == p.nx - 1) st_at(uo, uidx(p, x + 1, y, z), static_cast<float>(uR));
+    st_at(vo, vidx(p, x, y, z), static_cast<float>(vD));
+    if (y == p.ny - 1) st_at(vo, vidx(p, x, y + 1, z), static_cast<float>(vU));
+    st_at(wo, widx(p, x, y, z), static_cast<float>(wB));
+    if (z == p.nz - 1) st_at(wo, widx(p, x, y, z + 1), static_cast<float>(wF));
     const float div = static_cast<float>(uR - uL + vU - vD + wF - wB);
-    const size_t idx = sidx(p, x, y, z);
-    D[idx] = -div;
-    P[idx] = 0.0f;
+    const Idx idx = sidx(p, x, y, z);
+    st_at(D, idx, -div);
+    st_at(P, idx, 0.0f);
 }
 
 __global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
@@ -435,17 +435,17 @@ __device__ __forceinline__ void sor_half_at(float* __restrict__ P, const float* 
                                             int tx, int y, int z) {
     const int x = 2 * tx + ((parity ^ y ^ z) & 1);
     if (x >= nx || y >= ny || z >= nz) return;
-    const size_t row = static_cast<size_t>(nx);
-    const size_t plane = row * ny;
-    const size_t idx = static_cast<size_t>(z) * plane + static_cast<size_t>(y) * row + x;
-    const float pc = P[idx];
-    const float pL = x > 0 ? P[idx - 1] : pc;
-    const float pR = x < nx - 1 ? P[idx + 1] : pc;
-    const float pD = y > 0 ? P[idx - row] : pc;
-    const float pU = y < ny - 1 ? P[idx + row] : pc;
-    const float pB = z > 0 ? P[idx - plane] : pc;
-    const float pF = z < nz - 1 ? P[idx + plane] : pc;
-    P[idx] = sor_update(D[idx], pc, pL, pR, pD, pU, pB, pF, omega);
+    const Idx row = static_cast<Idx>(nx);
+    const Idx plane = row * static_cast<Idx>(ny);
+    const Idx idx = static_cast<Idx>(z) * plane + static_cast<Idx>(y) * row + static_cast<Idx>(x);
+    const float pc = ld_at(P, idx);
+    const float pL = x > 0 ? ld_at(P, idx - 1) : pc;
+    const float pR = x < nx - 1 ? ld_at(P, idx + 1) : pc;
+    const float pD = y > 0 ? ld_at(P, idx - row) : pc;
+    const float pU = y < ny - 1 ? ld_at(P, idx + row) : pc;
+    const float pB = z > 0 ? ld_at(P, idx - plane) : pc;
+    const float pF = z < nz - 1 ? ld_at(P, idx + plane) : pc;
+    st_at(P, idx, sor_update(ld_at(D, idx), pc, pL, pR, pD, pU, pB, pF, omega));
 }
 
 __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const float* __restrict__ D, int parity, float omega, int nx, int ny, int nz) {
@@ -475,7 +475,11 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
 // near 35 VGPRs. The tile shapes trade halo redundancy against occupancy:
 // 16^3 tiles (55 KB of LDS, two workgroups per CU) suit small grids, 24x24
 // tiles with a 32x32 loaded face (1024 threads, one or two per CU) larger ones.
-template <int T, int K, int TZ>
+// Unroll depth of the sweep loop: 4 for deep tiles (one workgroup per CU,
+// register room to spare), 2 where two workgroups share a CU.
+constexpr int sor_unroll(int tz) { return tz >= 24 ? 4 : 2; }
+
+template <int T, int K, int TZ, int UNR = sor_unroll(TZ)>
 __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D, float omega, int h0,
                                          int count, int nx, int ny, int nz, float* __restrict__ sp, uint3 blk) {
     constexpr int E = T + 2 * K;    // loaded edge in x and y
@@ -520,7 +524,7 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
             const int lz0 = az + ((az ^ zg) & 1);
             int i = (lz0 * E + ly) * E + lx;
             const float* dg = D + (static_cast<size_t>(bz + lz0) * plane + gy * row + gx);
-#pragma unroll 2
+#pragma unroll UNR
             for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, dg += 2 * plane) {
                 const int gz = bz + lz;
                 const float dv = *dg;
@@ -561,19 +565,19 @@ __device__ __forceinline__ void project_at(float* __restrict__ U, float* __restr
                                            const DevParams& p, int x, int y, int z) {
     if (x > p.nx || y > p.ny || z > p.nz) return;
     if (y < p.ny && z < p.nz) {
-        const size_t f = uidx(p, x, y, z);
-        if (x == 0 || x == p.nx) U[f] = 0.0f;
-        else U[f] -= P[sidx(p, x, y, z)] - P[sidx(p, x - 1, y, z)];
+        const Idx f = uidx(p, x, y, z);
+        if (x == 0 || x == p.nx) st_at(U, f, 0.0f);
+        else U[f] -= ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x - 1, y, z));
     }
     if (x < p.nx && z < p.nz) {
-        const size_t f = vidx(p, x, y, z);
-        if (y == 0) V[f] = 0.0f;
-        else if (y < p.ny) V[f] -= P[sidx(p, x, y, z)] - P[sidx(p, x, y - 1, z)];
+        const Idx f = vidx(p, x, y, z);
+        if (y == 0) st_at(V, f, 0.0f);
+        else if (y < p.ny) V[f] -= ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x, y - 1, z));
     }
     if (x < p.nx && y < p.ny) {
-        const size_t f = widx(p, x, y, z);
-        if (z == 0 || z == p.nz) W[f] = 0.0f;
-        else W[f] -= P[sidx(p, x, y, z)] - P[sidx(p, x, y, z - 1)];
+        const Idx f = widx(p, x, y, z);
+        if (z == 0 || z == p.nz) st_at(W, f, 0.0f);
+        else W[f] -= ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x, y, z - 1));
     }
 }
 
@@ -591,7 +595,7 @@ __global__ __launch_bounds__(256) void k_reset(Scalars<B> S, float* __restrict__
     const int y = blk.y * blockDim.y + threadIdx.y;
     const int z = blk.z * blockDim.z + threadIdx.z;
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
-    const size_t idx = sidx(p, x, y, z);
+    const Idx idx = sidx(p, x, y, z);
     const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
     const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
     const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
@@ -624,8 +628,8 @@ __global__ __launch_bounds__(256) void k_reset(Scalars<B> S, float* __restrict__
     S.template st<PROGRESS>(idx, prog);
     S.template st<TURB>(idx, k);
     S.template st<SOPT>(idx, so);
-    P[idx] = 0.0f;
-    D[idx] = 0.0f;
+    st_at(P, idx, 0.0f);
+    st_at(D, idx, 0.0f);
 }
 
 // ------------------------------------------------------------ metrics ----
@@ -645,14 +649,14 @@ __global__ __launch_bounds__(256) void k_metrics(double* __restrict__ partial, S
         const float soot = S.template ld<SOOT>(idx), ch = S.template ld<CHARF>(idx), ash = S.template ld<ASH>(idx);
         const float pyro = S.template ld<PYRO>(idx), prog = S.template ld<PROGRESS>(idx), turb = S.template ld<TURB>(idx);
         const float so = S.template ld<SOPT>(idx);
-        const float divBefore = fabsf(D[idx]);
-        const float div = U[uidx(p, x + 1, y, z)] - U[uidx(p, x, y, z)] + V[vidx(p, x, y + 1, z)] - V[vidx(p, x, y, z)] + W[widx(p, x, y, z + 1)] -
-                          W[widx(p, x, y, z)];
+        const float divBefore = fabsf(ld_at(D, idx));
+        const float div = ld_at(U, uidx(p, x + 1, y, z)) - ld_at(U, uidx(p, x, y, z)) + ld_at(V, vidx(p, x, y + 1, z)) - ld_at(V, vidx(p, x, y, z)) + ld_at(W, widx(p, x, y, z + 1)) -
+                          ld_at(W, widx(p, x, y, z));
         const float divAfter = fabsf(-div);
-        const float pc = P[idx];
-        const double resid = static_cast<double>(D[idx]) + P[sidx(p, max(x - 1, 0), y, z)] + P[sidx(p, min(x + 1, p.nx - 1), y, z)] +
-                             P[sidx(p, x, max(y - 1, 0), z)] + P[sidx(p, x, min(y + 1, p.ny - 1), z)] + P[sidx(p, x, y, max(z - 1, 0))] +
-                             P[sidx(p, x, y, min(z + 1, p.nz - 1))] - 6.0 * pc;
+        const float pc = ld_at(P, idx);
+        const double resid = static_cast<double>(ld_at(D, idx)) + ld_at(P, sidx(p, max(x - 1, 0), y, z)) + ld_at(P, sidx(p, min(x + 1, p.nx - 1), y, z)) +
+                             ld_at(P, sidx(p, x, max(y - 1, 0), z)) + ld_at(P, sidx(p, x, min(y + 1, p.ny - 1), z)) + ld_at(P, sidx(p, x, y, max(z - 1, 0))) +
+                             ld_at(P, sidx(p, x, y, min(z + 1, p.nz - 1))) - 6.0 * pc;
         const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
         const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
         const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);

@@ -53,13 +53,22 @@ def test_scene_bytes_accounting():
     assert fp32 - b.value == 6 * 2 * cells  # heat, fuel, soot in both buffers at 2 B
 
 
-@pytest.mark.parametrize("desc", [(3, 8, 8, 16, 16, 0, 0), (8, 8, 8, 0, 16, 0, 0), (8, 8, 8, 16, 16, 7, 0)])
+@pytest.mark.parametrize("desc", [(3, 8, 8, 16, 16, 0, 0), (8, 8, 8, 0, 16, 0, 0), (8, 8, 8, 16, 16, 7, 0),
+                                  (1024, 1024, 1023, 16, 16, 0, 0)])
 def test_create_rejects_bad_descriptors_without_touching_the_device(desc):
     d = firesim.SceneDesc(*desc)
     h = ctypes.c_void_p()
     assert LIB.fs_create(ctypes.byref(d), ctypes.byref(h)) == 1  # FS_ERR_INVALID_ARGUMENT
     assert not h.value
     assert LIB.fs_last_global_error()
+
+
+def test_grid_cap_keeps_every_field_under_2_30_values():
+    # kernels address fields with 32-bit byte offsets: (nx+1)(ny+1)(nz+1) <= 2^30
+    h = ctypes.c_void_p()
+    over = firesim.SceneDesc(1024, 1024, 1023, 16, 16, 0, 0)  # 1025 * 1025 * 1024 > 2^30
+    assert LIB.fs_create(ctypes.byref(over), ctypes.byref(h)) == 1
+    assert b"2^30" in LIB.fs_last_global_error()
 
 
 def test_null_handles_are_errors():
