@@ -272,7 +272,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: runs on one GPU only; --gpus {args.gpus} is not supported "
               "(scenes shard one per GPU with no exchange, see the design notes).", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:

@@ -97,7 +97,7 @@ bool step_and_render(std::uint32_t* bgra, const FireSettings& settings, FireCuda
                      bool d3d) {
     if (!g_scene) return fail("Renderer is not initialized.");
     if (render && !d3d && bgra == nullptr) return fail("Output pixel pointer was null.");
-    if (render && d3d) return fail("D3D11 interop is not available in this build; render to a host buffer with fireCudaStepAndRender.");
+    if (render && d3d) return fail("D3D11 interop is not available in this library; render to a host buffer with fireCudaStepAndRender.");
     if (!render && !advance) return fail("Step-only path requires simulation advancement.");
     if (!apply(settings)) return false;
     // In-call reset (:3136-3143, :3181, :3053-3055): the reference advances
@@ -220,10 +220,10 @@ bool fireCudaStepAndRenderMeasured(std::uint32_t* bgraPixels, const FireSettings
     return step_and_render(bgraPixels, settings, metrics, true, true, false);
 }
 
-bool fireCudaSelectDeviceForD3D11(void*) { return fail("D3D11 interop is not available in this build."); }
-bool fireCudaRegisterD3D11Texture(void*) { return fail("D3D11 interop is not available in this build."); }
-bool fireCudaRegisterD3D11TextureSlot(int, void*) { return fail("D3D11 interop is not available in this build."); }
-bool fireCudaSetD3D11TextureSlot(int) { return fail("D3D11 interop is not available in this build."); }
+bool fireCudaSelectDeviceForD3D11(void*) { return fail("D3D11 interop is not available in this library."); }
+bool fireCudaRegisterD3D11Texture(void*) { return fail("D3D11 interop is not available in this library."); }
+bool fireCudaRegisterD3D11TextureSlot(int, void*) { return fail("D3D11 interop is not available in this library."); }
+bool fireCudaSetD3D11TextureSlot(int) { return fail("D3D11 interop is not available in this library."); }
 
 // Step only, no render (src/fire_cuda.cu:3624-3626).
 bool fireCudaStepD3D11(const FireSettings& settings) { return step_and_render(nullptr, settings, nullptr, true, false, false); }

@@ -5,7 +5,7 @@
 // the reference's per-frame settings, writes the same three artefacts with the
 // same columns and keys (validation-metrics.csv, validation-report.json,
 // validation-frame.bmp; validation-app-frame.bmp holds the same frame, as the
-// display overlay is not part of this build), applies the reference's
+// display overlay is not part of this library), applies the reference's
 // stability checks and any --targets= envelope CSV ("metric,min,max" rows),
 // and exits 0 when validation passes, 4 when it fails, 2/3 on setup errors.
 //
@@ -13,7 +13,7 @@
 // --scene= --pool-fire-calibration --dataset-id= --manifest=, plus
 // --assets= (scene contract directory, default assets/fire-scenes). The
 // real-burn calibration comparison (--calibration=, --geometry=) is not part
-// of this build: providing either marks the run as failed, as the reference
+// of this library: providing either marks the run as failed, as the reference
 // does for an unreadable calibration file.
 #include <algorithm>
 #include <cmath>
