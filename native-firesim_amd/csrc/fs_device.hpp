@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace fs {
 
 enum ScalarId { HEAT = 0, FUEL, OXYGEN, SOOT, CHARF, ASH, PYRO, PROGRESS, TURB, SOPT, NSCALAR };
@@ -48,17 +50,33 @@ struct DevParams {
 // Element index into a device array. Scene creation rejects grids with more
 // than 2^30 values in any field ((nx+1)(ny+1)(nz+1) <= 2^30), so an element's
 // byte offset fits 32 bits. ld_at/st_at address an element as uniform base +
-// 32-bit byte offset, which the loads take directly (no 64-bit address math).
+// 32-bit byte offset in the global address space: the loads then take the
+// offset directly (no 64-bit address math), and pointers read from a batch
+// table (which the compiler cannot place) still give global, not flat,
+// accesses. Every array they touch is device memory of a scene.
 using Idx = uint32_t;
 constexpr double kMaxFieldValues = 1073741824.0;  // 2^30
 
+// (Vector types such as float4 cannot bind address-space-qualified
+// references; they take the generic path.)
+#define FS_GLOBAL __attribute__((address_space(1)))
 template <class T>
 __device__ __forceinline__ T ld_at(const T* __restrict__ base, Idx i) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<Idx>(sizeof(T)));
+    if constexpr (std::is_arithmetic_v<T>) {
+        const FS_GLOBAL char* b = (const FS_GLOBAL char*)base;
+        return *(const FS_GLOBAL T*)(b + i * static_cast<Idx>(sizeof(T)));
+    } else {
+        return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * static_cast<Idx>(sizeof(T)));
+    }
 }
 template <class T>
 __device__ __forceinline__ void st_at(T* __restrict__ base, Idx i, T v) {
-    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<Idx>(sizeof(T))) = v;
+    if constexpr (std::is_arithmetic_v<T>) {
+        FS_GLOBAL char* b = (FS_GLOBAL char*)base;
+        *(FS_GLOBAL T*)(b + i * static_cast<Idx>(sizeof(T))) = v;
+    } else {
+        *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * static_cast<Idx>(sizeof(T))) = v;
+    }
 }
 
 // Scalar field set. When BF16 is true, heat, fuel and soot are stored as

@@ -34,7 +34,7 @@ __device__ __forceinline__ void light_build_at(float4* __restrict__ L, float* __
     const float gy = sample_scalar<HEAT, B>(S, p, u, v + cellY, w) - sample_scalar<HEAT, B>(S, p, u, v - cellY, w);
     const float gz = sample_scalar<HEAT, B>(S, p, u, v, w + cellZ) - sample_scalar<HEAT, B>(S, p, u, v, w - cellZ);
     const float front = sqrtf(gx * gx + gy * gy + gz * gz) + fuel * oxygen * 0.035f + progress * 0.16f;
-    const float t = p.clock[0];
+    const float t = ld_at(p.clock, 0u);
     const float fine = fbm3(u * 34.0f + t * 0.45f, v * 42.0f - t * 0.82f, w * 34.0f + t * 0.22f, p.saltNoise3);
     const float shear = fbm3(u * 12.0f - t * 0.38f, v * 11.0f - t * 0.62f, w * 13.0f + t * 0.20f, p.saltNoise3);
     const float reactionFront = sstep(0.035f, 0.54f, front + pyro * 0.10f);
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(256) void k_render(float4* __restrict__ out, Scalar
     float trans = 1.0f, flameGlow = 0.0f, smokeOcc = 0.0f;
     float dbgFlame = 0.0f, dbgOD = 0.0f, dbgTemp = 0.0f, dbgFuel = 0.0f, dbgVel = 0.0f;
     const int sid = p.sceneId;
-    const float t = p.clock[1];
+    const float t = ld_at(p.clock, 1u);
     if (intersect_box(cam.eye, rd, boxNear, boxFar)) {
         const int steps = min(max(p.raymarchSteps, 24), 256);
         const float t0 = fmaxf(0.0f, boxNear);
@@ -628,7 +628,7 @@ __global__ __launch_bounds__(kMaxEmbers) void k_ember_prepare(Ember* __restrict_
     e.active = 0;
     if (i >= kMaxEmbers) return;
     const int count = min(max(p.emberCount, 0), kMaxEmbers);
-    const float time = p.clock[1];
+    const float time = ld_at(p.clock, 1u);
     const int sid = p.sceneId;
     const float baseGlow = sat(sample_scalar<HEAT, B>(S, p, 0.50f, 0.06f, 0.50f) * 0.42f);
     const float sparkGate = sstep(0.02f, 0.30f, baseGlow);

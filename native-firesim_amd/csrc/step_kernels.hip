@@ -123,7 +123,7 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
     const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
     const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
     const float dt = p.dt;
-    const float time = p.clock[0];
+    const float time = ld_at(p.clock, 0u);
     const int sid = p.sceneId;
     const float h0 = in.template ld<HEAT>(idx), f0 = in.template ld<FUEL>(idx), o0 = in.template ld<OXYGEN>(idx);
     const float s0 = in.template ld<SOOT>(idx), ch0 = in.template ld<CHARF>(idx), a0 = in.template ld<ASH>(idx);
@@ -337,7 +337,7 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
     const float curlY = (uR - uL) - (wF - wB);
     const float curlZ = (vU - vD) - (uR - uL);
     const float curlMag = fabsf(curlX) + fabsf(curlY) + fabsf(curlZ);
-    const float t = p.clock[0];
+    const float t = ld_at(p.clock, 0u);
     const float noiseA = fbm2(u * 9.0f + w * 3.7f + t * 0.22f, v * 13.0f - t * 0.31f, p.saltNoise2) - 0.5f;
     const float noiseB = fbm2(w * 10.0f - t * 0.19f, u * 8.0f + v * 5.5f + t * 0.27f, p.saltNoise2) - 0.5f;
     const float thermal = sstep(0.04f, 4.20f, h + s * 0.28f);
@@ -490,16 +490,16 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
     const int tid = threadIdx.x;
     const int ox = blk.x * T, oy = blk.y * T, oz = blk.z * TZ;
     const int bx = ox - K, by = oy - K, bz = oz - K;  // global coordinate of local 0
-    const size_t row = static_cast<size_t>(nx), plane = row * ny;
+    const Idx row = static_cast<Idx>(nx), plane = row * static_cast<Idx>(ny);
     {
         const int gx = bx + tid % E, gy = by + tid / E;
         const bool colOk = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
         float v[EZ];
-        const float* src = P + (colOk ? gy * row + gx : 0);
+        const Idx col = colOk ? static_cast<Idx>(gy) * row + static_cast<Idx>(gx) : 0u;
 #pragma unroll
         for (int lz = 0; lz < EZ; ++lz) {
             const int gz = bz + lz;
-            v[lz] = (colOk && gz >= 0 && gz < nz) ? src[gz * plane] : 0.0f;
+            v[lz] = (colOk && gz >= 0 && gz < nz) ? ld_at(P, static_cast<Idx>(gz) * plane + col) : 0.0f;
         }
 #pragma unroll
         for (int lz = 0; lz < EZ; ++lz) sp[lz * NT + tid] = v[lz];
@@ -523,11 +523,11 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
             const int oL = gx > 0 ? -1 : 0, oR = gx < nx - 1 ? 1 : 0;
             const int lz0 = az + ((az ^ zg) & 1);
             int i = (lz0 * E + ly) * E + lx;
-            const float* dg = D + (static_cast<size_t>(bz + lz0) * plane + gy * row + gx);
+            Idx di = static_cast<Idx>(bz + lz0) * plane + static_cast<Idx>(gy) * row + static_cast<Idx>(gx);
 #pragma unroll UNR
-            for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, dg += 2 * plane) {
+            for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, di += 2 * plane) {
                 const int gz = bz + lz;
-                const float dv = *dg;
+                const float dv = ld_at(D, di);
                 const float pc = sp[i];
                 const float pL = sp[i + oL];
                 const float pR = sp[i + oR];
@@ -544,11 +544,11 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
         const int tx = tid % T, ty = tid / T;
         const int gx = ox + tx, gy2 = oy + ty;
         if (gx < nx && gy2 < ny) {
-            float* dst = Pout + gy2 * row + gx;
+            const Idx dst = static_cast<Idx>(gy2) * row + static_cast<Idx>(gx);
             const int src = (K * E + ty + K) * E + tx + K;
 #pragma unroll
             for (int tz = 0; tz < TZ; ++tz)
-                if (oz + tz < nz) dst[(oz + tz) * plane] = sp[src + tz * NT];
+                if (oz + tz < nz) st_at(Pout, static_cast<Idx>(oz + tz) * plane + dst, sp[src + tz * NT]);
         }
     }
 }
@@ -567,17 +567,17 @@ __device__ __forceinline__ void project_at(float* __restrict__ U, float* __restr
     if (y < p.ny && z < p.nz) {
         const Idx f = uidx(p, x, y, z);
         if (x == 0 || x == p.nx) st_at(U, f, 0.0f);
-        else U[f] -= ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x - 1, y, z));
+        else st_at(U, f, ld_at(U, f) - (ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x - 1, y, z))));
     }
     if (x < p.nx && z < p.nz) {
         const Idx f = vidx(p, x, y, z);
         if (y == 0) st_at(V, f, 0.0f);
-        else if (y < p.ny) V[f] -= ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x, y - 1, z));
+        else if (y < p.ny) st_at(V, f, ld_at(V, f) - (ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x, y - 1, z))));
     }
     if (x < p.nx && y < p.ny) {
         const Idx f = widx(p, x, y, z);
         if (z == 0 || z == p.nz) st_at(W, f, 0.0f);
-        else W[f] -= ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x, y, z - 1));
+        else st_at(W, f, ld_at(W, f) - (ld_at(P, sidx(p, x, y, z)) - ld_at(P, sidx(p, x, y, z - 1))));
     }
 }
 
