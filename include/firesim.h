@@ -196,12 +196,14 @@ typedef struct fs_stage_times {
 enum fs_sor_kernel {
     FS_SOR_HALF_SWEEP = 0,      /* one launch per half-sweep, as the reference (:3280-3284) */
     FS_SOR_TB_T16_K4 = 1,       /* 16^3 tiles, 4 half-sweeps per launch */
-    FS_SOR_TB_T24_K4 = 2,       /* 24^3 tiles, 4 half-sweeps per launch (default) */
+    FS_SOR_TB_T24_K4 = 2,       /* 24^3 tiles, 4 half-sweeps per launch */
     FS_SOR_TB_T24_Z8_K4 = 3,    /* 24x24x8 tiles */
     FS_SOR_TB_T24_Z16_K4 = 4,   /* 24x24x16 tiles */
     FS_SOR_TB_T24_Z28_K4 = 5,   /* 24x24x28 tiles */
-    FS_SOR_TB_T24_Z10_K4 = 6,   /* 24x24x10 tiles */
-    FS_SOR_KERNEL_COUNT = 7
+    FS_SOR_TB_T24_Z10_K4 = 6,   /* 24x24x10 tiles, two workgroups per CU (default) */
+    FS_SOR_TB_T24_Z12_K4 = 7,   /* 24x24x12 tiles, two workgroups per CU */
+    FS_SOR_TB_T24_Z6_K4 = 8,    /* 24x24x6 tiles, for small grids */
+    FS_SOR_KERNEL_COUNT = 9
 };
 
 /* Launch-shape choices of a scene (picked by the autotuner). */

@@ -26,7 +26,7 @@ struct fs_batch {
     hipStream_t stream = nullptr;
     std::vector<fs_scene*> members;  // in insertion order
     bool bf16 = false;
-    int sorKernel = 2;
+    int sorKernel = FS_SOR_TB_T24_Z10_K4;
     bool useGraph = true;
     int grid = 0;
     BatchTable* devTable = nullptr;

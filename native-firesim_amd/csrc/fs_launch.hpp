@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "firesim.h"
 #include "fs_device.hpp"
 
 namespace fs {
@@ -44,7 +45,7 @@ struct DeviceScene {
     double* metricsOut = nullptr;  // kMetricSlots doubles
     float* clock = nullptr;        // [simulation time, render time]
     Ember* embers = nullptr;       // kMaxEmbers sparks of the last render
-    int sorKernel = 2;           // enum fs_sor_kernel
+    int sorKernel = FS_SOR_TB_T24_Z10_K4;  // enum fs_sor_kernel
 };
 
 // ------------------------------------------------------------ batches ----

@@ -313,7 +313,7 @@ int fs_server_create(const fs_server_desc* desc, fs_server** out) {
         delete sv;
         return st;
     }
-    fs_tuning t{desc->sor_kernel < 0 ? FS_SOR_TB_T24_K4 : desc->sor_kernel, desc->use_graph};
+    fs_tuning t{desc->sor_kernel < 0 ? FS_SOR_TB_T24_Z10_K4 : desc->sor_kernel, desc->use_graph};
     fs_batch_set_tuning(sv->batch, &t);
     sv->tuned = desc->sor_kernel >= 0;
     *out = sv;

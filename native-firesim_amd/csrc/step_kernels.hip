@@ -469,8 +469,9 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
 // (x = t % E, y = t / E) and walks z, so addresses only increment. For the
 // half-sweeps, thread t owns the colour cell pair k = t % (E/2) of row
 // y = (t / (E/2)) % E and every other z plane starting at z = t / (E^2/2);
-// along its planes the colour parity of its x position is fixed, so the wall
-// clamps in x and y become fixed LDS offsets (offset 0 reads the cell itself).
+// along its planes the colour parity of its x position is fixed. Every
+// neighbour is read at a fixed LDS offset: at a domain wall the slot past the
+// wall holds a copy of the wall cell (the reference's clamped-index neighbour).
 // d is read from global memory (L2) inside the sweep, which keeps the kernel
 // near 35 VGPRs. The tile shapes trade halo redundancy against occupancy:
 // 16^3 tiles (55 KB of LDS, two workgroups per CU) suit small grids, 24x24
@@ -492,15 +493,15 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
     const int bx = ox - K, by = oy - K, bz = oz - K;  // global coordinate of local 0
     const Idx row = static_cast<Idx>(nx), plane = row * static_cast<Idx>(ny);
     {
-        const int gx = bx + tid % E, gy = by + tid / E;
-        const bool colOk = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        // Box cells outside the domain take the value of the nearest wall
+        // cell (indices clamped per axis). The slot just past a wall then
+        // holds that wall cell's value: the reference's missing neighbour
+        // (the cell itself, by its clamped indices) read at a fixed offset.
+        const int gx = min(max(bx + tid % E, 0), nx - 1), gy = min(max(by + tid / E, 0), ny - 1);
         float v[EZ];
-        const Idx col = colOk ? static_cast<Idx>(gy) * row + static_cast<Idx>(gx) : 0u;
+        const Idx col = static_cast<Idx>(gy) * row + static_cast<Idx>(gx);
 #pragma unroll
-        for (int lz = 0; lz < EZ; ++lz) {
-            const int gz = bz + lz;
-            v[lz] = (colOk && gz >= 0 && gz < nz) ? ld_at(P, static_cast<Idx>(gz) * plane + col) : 0.0f;
-        }
+        for (int lz = 0; lz < EZ; ++lz) v[lz] = ld_at(P, static_cast<Idx>(min(max(bz + lz, 0), nz - 1)) * plane + col);
 #pragma unroll
         for (int lz = 0; lz < EZ; ++lz) sp[lz * NT + tid] = v[lz];
     }
@@ -510,7 +511,6 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
     const int rx1 = min(E, nx - bx), ry1 = min(E, ny - by), rz1 = min(EZ, nz - bz);
     const int k = tid % HX, ly = (tid / HX) % E, zg = tid / (HX * E);
     const int gy = by + ly;
-    const int oD = gy > 0 ? -E : 0, oU = gy < ny - 1 ? E : 0;
     for (int s = 0; s < count; ++s) {
         const int parity = (h0 + s) & 1;
         const int ax = bx < 0 ? rx0 : s + 1, ay = by < 0 ? ry0 : s + 1, az = bz < 0 ? rz0 : s + 1;
@@ -520,22 +520,28 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
         const int lx = 2 * k + ((parity ^ (bx + gy + bz + zg)) & 1);
         if (lx >= ax && lx < ex && ly >= ay && ly < ey) {
             const int gx = bx + lx;
-            const int oL = gx > 0 ? -1 : 0, oR = gx < nx - 1 ? 1 : 0;
             const int lz0 = az + ((az ^ zg) & 1);
-            int i = (lz0 * E + ly) * E + lx;
+            const int i0 = (lz0 * E + ly) * E + lx;
+            int i = i0;
             Idx di = static_cast<Idx>(bz + lz0) * plane + static_cast<Idx>(gy) * row + static_cast<Idx>(gx);
 #pragma unroll UNR
-            for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, di += 2 * plane) {
-                const int gz = bz + lz;
-                const float dv = ld_at(D, di);
-                const float pc = sp[i];
-                const float pL = sp[i + oL];
-                const float pR = sp[i + oR];
-                const float pD = sp[i + oD];
-                const float pU = sp[i + oU];
-                const float pB = sp[i + (gz > 0 ? -NT : 0)];
-                const float pF = sp[i + (gz < nz - 1 ? NT : 0)];
-                sp[i] = sor_update(dv, pc, pL, pR, pD, pU, pB, pF, omega);
+            for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, di += 2 * plane)
+                sp[i] = sor_update(ld_at(D, di), sp[i], sp[i - 1], sp[i + 1], sp[i - E], sp[i + E], sp[i - NT], sp[i + NT], omega);
+            // Wall cells refresh the slot past their wall, which only they
+            // read (in a later half-sweep).
+            const int gxo = gx == 0 ? -1 : (gx == nx - 1 ? 1 : 0);
+            const int gyo = gy == 0 ? -E : (gy == ny - 1 ? E : 0);
+            if (gxo != 0 || gyo != 0) {
+                for (int j = i0; j < i; j += 2 * NT) {
+                    const float r = sp[j];
+                    if (gxo != 0) sp[j + gxo] = r;
+                    if (gyo != 0) sp[j + gyo] = r;
+                }
+            }
+            if (i > i0) {
+                if (bz + lz0 == 0) sp[i0 - NT] = sp[i0];
+                const int il = i - 2 * NT;  // last plane updated
+                if (bz + lz0 + (il - i0) / NT == nz - 1) sp[il + NT] = sp[il];
             }
         }
         __syncthreads();
@@ -912,6 +918,8 @@ static SorShape sor_shape(int sorKernel) {
         case 4: return {24, 16, 4};
         case 5: return {24, 28, 4};
         case 6: return {24, 10, 4};
+        case 7: return {24, 12, 4};
+        case 8: return {24, 6, 4};
         default: return {0, 0, 1};  // plain half-sweep kernel
     }
 }
@@ -941,6 +949,8 @@ hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
         case 4: launch_sor_tb<24, 4, 16>(s, p, st); break;
         case 5: launch_sor_tb<24, 4, 28>(s, p, st); break;
         case 6: launch_sor_tb<24, 4, 10>(s, p, st); break;
+        case 7: launch_sor_tb<24, 4, 12>(s, p, st); break;
+        case 8: launch_sor_tb<24, 4, 6>(s, p, st); break;
         default: {
             const dim3 b = cell_block();
             const dim3 g = sor_grid(0, s.nx, s.ny, s.nz);
@@ -1022,6 +1032,8 @@ hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sor
         case 4: launch_batch_sor_tb<24, 4, 16>(table, sorLaunches, grid, st); break;
         case 5: launch_batch_sor_tb<24, 4, 28>(table, sorLaunches, grid, st); break;
         case 6: launch_batch_sor_tb<24, 4, 10>(table, sorLaunches, grid, st); break;
+        case 7: launch_batch_sor_tb<24, 4, 12>(table, sorLaunches, grid, st); break;
+        case 8: launch_batch_sor_tb<24, 4, 6>(table, sorLaunches, grid, st); break;
         default:
             for (int j = 0; j < sorLaunches; ++j) kb_sor_half<<<grid, cell_block(), 0, st>>>(table, j);
     }

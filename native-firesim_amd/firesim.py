@@ -31,7 +31,8 @@ STAGE_ADVECT_VELOCITY, STAGE_ADVECT_REACT, STAGE_FORCE, STAGE_PRESSURE, STAGE_PR
 STAGE_NAMES = ["advect_velocity", "advect_react", "force", "pressure", "project", "light"]
 
 STORAGE_FP32, STORAGE_BF16 = 0, 1
-SOR_KERNELS = ["half_sweep", "tb_t16_k4", "tb_t24_k4", "tb_t24_z8_k4", "tb_t24_z16_k4", "tb_t24_z28_k4", "tb_t24_z10_k4"]
+SOR_KERNELS = ["half_sweep", "tb_t16_k4", "tb_t24_k4", "tb_t24_z8_k4", "tb_t24_z16_k4", "tb_t24_z28_k4", "tb_t24_z10_k4", "tb_t24_z12_k4",
+               "tb_t24_z6_k4"]
 SCENE_ROOM_TRAY, SCENE_CAMPFIRE, SCENE_GAS_BURNER, SCENE_EMITTERS_ONLY = 0, 1, 2, 3
 MAX_EMITTERS = 16
 
@@ -145,7 +146,7 @@ class Tuning(Structure):
 
 
 class AutotuneReport(Structure):
-    _fields_ = [("sor_ms", c_double * 7), ("chosen", c_int32)]
+    _fields_ = [("sor_ms", c_double * len(SOR_KERNELS)), ("chosen", c_int32)]
 
 
 class StageTimes(Structure):
