@@ -524,9 +524,15 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
             const int i0 = (lz0 * E + ly) * E + lx;
             int i = i0;
             Idx di = static_cast<Idx>(bz + lz0) * plane + static_cast<Idx>(gy) * row + static_cast<Idx>(gx);
+            // The cell above plane lz (other colour, unchanged in this
+            // half-sweep) is the cell below plane lz + 2: read it once.
+            float pB = lz0 < ez ? sp[i0 - NT] : 0.0f;
 #pragma unroll UNR
-            for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, di += 2 * plane)
-                sp[i] = sor_update(ld_at(D, di), sp[i], sp[i - 1], sp[i + 1], sp[i - E], sp[i + E], sp[i - NT], sp[i + NT], omega);
+            for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, di += 2 * plane) {
+                const float pF = sp[i + NT];
+                sp[i] = sor_update(ld_at(D, di), sp[i], sp[i - 1], sp[i + 1], sp[i - E], sp[i + E], pB, pF, omega);
+                pB = pF;
+            }
             // Wall cells refresh the slot past their wall, which only they
             // read (in a later half-sweep).
             const int gxo = gx == 0 ? -1 : (gx == nx - 1 ? 1 : 0);
