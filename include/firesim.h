@@ -203,7 +203,9 @@ enum fs_sor_kernel {
     FS_SOR_TB_T24_Z10_K4 = 6,   /* 24x24x10 tiles, two workgroups per CU (default) */
     FS_SOR_TB_T24_Z12_K4 = 7,   /* 24x24x12 tiles, two workgroups per CU */
     FS_SOR_TB_T24_Z6_K4 = 8,    /* 24x24x6 tiles, for small grids */
-    FS_SOR_KERNEL_COUNT = 9
+    FS_SOR_TB_T26_Z13_K3 = 9,   /* 26x26x13 tiles, 3 half-sweeps per launch (5 tiles span 128 cells) */
+    FS_SOR_TB_T26_Z9_K3 = 10,   /* 26x26x9 tiles, 3 half-sweeps per launch, two workgroups per CU */
+    FS_SOR_KERNEL_COUNT = 11
 };
 
 /* Launch-shape choices of a scene (picked by the autotuner). */
