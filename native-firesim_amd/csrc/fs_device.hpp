@@ -151,21 +151,65 @@ __device__ __forceinline__ float hash3(int x, int y, int z, uint32_t salt) {
     return unit24(pcg(pcg(pcg(salt + static_cast<uint32_t>(z)) + static_cast<uint32_t>(y)) + static_cast<uint32_t>(x)));
 }
 
-// Value noise, reference interpolation src/fire_cuda.cu:390-399.
-__device__ inline float vnoise2(float px, float py, uint32_t salt) {
-    const float fx0 = floorf(px), fy0 = floorf(py);
-    const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
-    const float fx = px - fx0, fy = py - fy0;
-    // one row hash per lattice y, shared by both x corners
-    const uint32_t r0 = pcg(salt + static_cast<uint32_t>(iy));
-    const uint32_t r1 = pcg(salt + static_cast<uint32_t>(iy + 1));
-    const float a = unit24(pcg(r0 + static_cast<uint32_t>(ix)));
-    const float b = unit24(pcg(r0 + static_cast<uint32_t>(ix + 1)));
-    const float c = unit24(pcg(r1 + static_cast<uint32_t>(ix)));
-    const float d = unit24(pcg(r1 + static_cast<uint32_t>(ix + 1)));
+// Value noise, reference interpolation src/fire_cuda.cu:390-399: the four
+// lattice values around (px, py), blended with smoothstep weights.
+__device__ __forceinline__ float lattice2(uint32_t rowHash, int x) { return unit24(pcg(rowHash + static_cast<uint32_t>(x))); }
+__device__ __forceinline__ float vblend2(float a, float b, float c, float d, float fx, float fy) {
     const float ux = fx * fx * (3.0f - 2.0f * fx);
     const float uy = fy * fy * (3.0f - 2.0f * fy);
     return lerpf(lerpf(a, b, ux), lerpf(c, d, ux), uy);
+}
+__device__ inline float vnoise2(float px, float py, uint32_t salt) {
+    const float fx0 = floorf(px), fy0 = floorf(py);
+    const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
+    // one row hash per lattice y, shared by both x corners
+    const uint32_t r0 = pcg(salt + static_cast<uint32_t>(iy));
+    const uint32_t r1 = pcg(salt + static_cast<uint32_t>(iy + 1));
+    return vblend2(lattice2(r0, ix), lattice2(r0, ix + 1), lattice2(r1, ix), lattice2(r1, ix + 1), px - fx0, py - fy0);
+}
+
+// vnoise2 evaluated by a whole wavefront at once, for lanes that are
+// neighbouring cells (all 64 lanes active). Each lattice value is then needed
+// by several lanes: lane L hashes one lattice point, and every lane takes its
+// four corners from the lanes that hashed them (ds_bpermute). The values are
+// those of vnoise2, bit for bit; when the lanes do not have the assumed shape
+// (all 64 active, a lattice row or column shared by all lanes, the others
+// within a span of 64) the per-lane hashes of vnoise2 are used instead.
+__device__ __forceinline__ bool full_wave() { return __builtin_amdgcn_read_exec() == ~0ull; }
+__device__ __forceinline__ float lane_value(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v)));
+}
+// Lanes on one lattice row (same floor(py)): the two row hashes are
+// wave-uniform (scalar unit), lane L hashes columns base + L of both rows.
+__device__ __forceinline__ float vnoise2_row_wave(float px, float py, uint32_t salt) {
+    const float fx0 = floorf(px), fy0 = floorf(py);
+    const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
+    const int iyw = __builtin_amdgcn_readfirstlane(iy);
+    if (!full_wave() || !__all(iy == iyw)) return vnoise2(px, py, salt);
+    const uint32_t r0 = pcg(salt + static_cast<uint32_t>(iyw));
+    const uint32_t r1 = pcg(salt + static_cast<uint32_t>(iyw + 1));
+    const int base = __builtin_amdgcn_readfirstlane(ix);
+    const int off = ix - base;
+    if (__all(off >= 0 && off < 63)) {
+        const float h0 = lattice2(r0, base + static_cast<int>(__lane_id())), h1 = lattice2(r1, base + static_cast<int>(__lane_id()));
+        return vblend2(lane_value(h0, off), lane_value(h0, off + 1), lane_value(h1, off), lane_value(h1, off + 1), px - fx0, py - fy0);
+    }
+    return vblend2(lattice2(r0, ix), lattice2(r0, ix + 1), lattice2(r1, ix), lattice2(r1, ix + 1), px - fx0, py - fy0);
+}
+// Lanes on one lattice column (same floor(px)): lane L hashes row base + L
+// and its two column values.
+__device__ __forceinline__ float vnoise2_col_wave(float px, float py, uint32_t salt) {
+    const float fx0 = floorf(px), fy0 = floorf(py);
+    const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
+    const int ixw = __builtin_amdgcn_readfirstlane(ix);
+    const int base = __builtin_amdgcn_readfirstlane(iy);
+    const int off = iy - base;
+    if (full_wave() && __all(ix == ixw && off >= 0 && off < 63)) {
+        const uint32_t r = pcg(salt + static_cast<uint32_t>(base + static_cast<int>(__lane_id())));
+        const float g0 = lattice2(r, ixw), g1 = lattice2(r, ixw + 1);
+        return vblend2(lane_value(g0, off), lane_value(g1, off), lane_value(g0, off + 1), lane_value(g1, off + 1), px - fx0, py - fy0);
+    }
+    return vnoise2(px, py, salt);
 }
 
 // src/fire_cuda.cu:401-421
@@ -194,18 +238,26 @@ __device__ inline float vnoise3(float px, float py, float pz, uint32_t salt) {
     return lerpf(lerpf(c00, c10, uy), lerpf(c01, c11, uy), uz);
 }
 
-// 5-octave fbm, src/fire_cuda.cu:423-432
-__device__ inline float fbm2(float px, float py, uint32_t salt) {
+// 5-octave fbm, src/fire_cuda.cu:423-432. WAVE selects the octave noise:
+// 0 per lane (vnoise2), 1 lanes on one lattice row, 2 lanes on one lattice
+// column (the wavefront forms above; same values).
+template <int WAVE>
+__device__ __forceinline__ float fbm2_t(float px, float py, uint32_t salt) {
     float sum = 0.0f, amp = 0.5f;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-        sum += amp * vnoise2(px, py, salt);
+        float n;
+        if constexpr (WAVE == 1) n = vnoise2_row_wave(px, py, salt);
+        else if constexpr (WAVE == 2) n = vnoise2_col_wave(px, py, salt);
+        else n = vnoise2(px, py, salt);
+        sum += amp * n;
         px = px * 2.03f + 17.7f;
         py = py * 2.01f - 9.2f;
         amp *= 0.5f;
     }
     return sum;
 }
+__device__ inline float fbm2(float px, float py, uint32_t salt) { return fbm2_t<0>(px, py, salt); }
 
 // 3-octave fbm, saturated, src/fire_cuda.cu:434-443
 __device__ inline float fbm3(float px, float py, float pz, uint32_t salt) {

@@ -317,13 +317,19 @@ template <bool B>
 __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, const Scalars<B>& S,
                                                  const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                  const DevParams& p, int x, int y, int z) {
+    // The two noise fields first, by every lane of the wavefront (lanes are
+    // consecutive x cells of one row, so noiseA's lattice row and noiseB's
+    // lattice column are shared by the whole wavefront).
+    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
+    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
+    const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const float t = ld_at(p.clock, 0u);
+    const float noiseA = fbm2_t<1>(u * 9.0f + w * 3.7f + t * 0.22f, v * 13.0f - t * 0.31f, p.saltNoise2) - 0.5f;
+    const float noiseB = fbm2_t<2>(w * 10.0f - t * 0.19f, u * 8.0f + v * 5.5f + t * 0.27f, p.saltNoise2) - 0.5f;
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
     const Idx idx = sidx(p, x, y, z);
     const int sid = p.sceneId;
     const float h = S.template ld<HEAT>(idx), s = S.template ld<SOOT>(idx), k = S.template ld<TURB>(idx), c = S.template ld<PROGRESS>(idx);
-    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
-    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
-    const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
     const int xm = max(x - 1, 0), xp = min(x + 1, p.nx - 1);
     const int ym = max(y - 1, 0), yp = min(y + 1, p.ny - 1);
     const int zm = max(z - 1, 0), zp = min(z + 1, p.nz - 1);
@@ -337,9 +343,6 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
     const float curlY = (uR - uL) - (wF - wB);
     const float curlZ = (vU - vD) - (uR - uL);
     const float curlMag = fabsf(curlX) + fabsf(curlY) + fabsf(curlZ);
-    const float t = ld_at(p.clock, 0u);
-    const float noiseA = fbm2(u * 9.0f + w * 3.7f + t * 0.22f, v * 13.0f - t * 0.31f, p.saltNoise2) - 0.5f;
-    const float noiseB = fbm2(w * 10.0f - t * 0.19f, u * 8.0f + v * 5.5f + t * 0.27f, p.saltNoise2) - 0.5f;
     const float thermal = sstep(0.04f, 4.20f, h + s * 0.28f);
     const float upperDrag = sstep(0.26f, 0.94f, v);
     const float lesGain = sqrtf(fmaxf(0.0f, k)) * (0.30f + c * 0.54f);
