@@ -288,29 +288,35 @@ struct Tri {
 __device__ __forceinline__ float sat_c(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 __device__ __forceinline__ double sat_c(double x) { return fmin(fmax(x, 0.0), 1.0); }
 
+// One axis of a sample: the clamped coordinate u in [0, 1] times the
+// multiplier s, its floor cell i0 (and i1 = i0 + 1, clamped to d - 1) and the
+// fraction. A trilinear setup is three of these, so a caller sampling several
+// arrays at one position (or one array at positions sharing a coordinate) can
+// form each axis once.
+struct Axis {
+    int i0, i1;
+    float t;
+};
 template <class T>
-__device__ __forceinline__ Tri tri_setup(int dx, int dy, int dz, T sx, T sy, T sz, T u, T v, T w) {
+__device__ __forceinline__ Axis axis_setup(int d, T s, T u) {
     u = sat_c(u);
-    v = sat_c(v);
-    w = sat_c(w);
-    const T fx = u * sx, fy = v * sy, fz = w * sz;
-    const T flx = floor(fx), fly = floor(fy), flz = floor(fz);
-    const int x0 = static_cast<int>(flx);
-    const int y0 = static_cast<int>(fly);
-    const int z0 = static_cast<int>(flz);
-    const int x1 = min(x0 + 1, dx - 1);
-    const int y1 = min(y0 + 1, dy - 1);
-    const int z1 = min(z0 + 1, dz - 1);
+    const T f = u * s;
+    const T fl = floor(f);
+    const int i0 = static_cast<int>(fl);
+    return Axis{i0, min(i0 + 1, d - 1), static_cast<float>(f - fl)};
+}
+// Corner offsets of an x-fastest array with rows of dx and planes of dx*dy.
+__device__ __forceinline__ Tri tri_of(const Axis& ax, const Axis& ay, const Axis& az, int dx, int dy) {
     Tri t;
-    t.tx = static_cast<float>(fx - flx);
-    t.ty = static_cast<float>(fy - fly);
-    t.tz = static_cast<float>(fz - flz);
+    t.tx = ax.t;
+    t.ty = ay.t;
+    t.tz = az.t;
     // corner (x0, y0, z0) and the steps to the +1 corners (0 where clamped)
     const Idx row = static_cast<Idx>(dx), plane = row * static_cast<Idx>(dy);
-    const Idx ox = static_cast<Idx>(x1 - x0);
-    const Idx oy = y1 != y0 ? row : 0u;
-    const Idx oz = z1 != z0 ? plane : 0u;
-    t.i000 = static_cast<Idx>(z0) * plane + static_cast<Idx>(y0) * row + static_cast<Idx>(x0);
+    const Idx ox = static_cast<Idx>(ax.i1 - ax.i0);
+    const Idx oy = ay.i1 != ay.i0 ? row : 0u;
+    const Idx oz = az.i1 != az.i0 ? plane : 0u;
+    t.i000 = static_cast<Idx>(az.i0) * plane + static_cast<Idx>(ay.i0) * row + static_cast<Idx>(ax.i0);
     t.i100 = t.i000 + ox;
     t.i010 = t.i000 + oy;
     t.i110 = t.i010 + ox;
@@ -319,6 +325,10 @@ __device__ __forceinline__ Tri tri_setup(int dx, int dy, int dz, T sx, T sy, T s
     t.i011 = t.i001 + oy;
     t.i111 = t.i011 + ox;
     return t;
+}
+template <class T>
+__device__ __forceinline__ Tri tri_setup(int dx, int dy, int dz, T sx, T sy, T sz, T u, T v, T w) {
+    return tri_of(axis_setup<T>(dx, sx, u), axis_setup<T>(dy, sy, v), axis_setup<T>(dz, sz, w), dx, dy);
 }
 
 __device__ __forceinline__ float tri_blend(const Tri& t, float c000, float c100, float c010, float c110, float c001, float c101,
