@@ -91,7 +91,8 @@ def cpu_baseline(name: str, seed: int, budget_s: float):
     dt = time.perf_counter() - t0
     cells = math.prod(cfg["grid"])
     o.close()
-    return {"value": cells * n / dt, "unit": "voxel-steps/s", "cores": threads, "kind": "port",
+    return {"value": cells * n / dt, "unit": "voxel-steps/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
+            "host_threads": os.cpu_count(),
             "sample": f"FP64 oracle, same {cfg['grid'][0]}x{cfg['grid'][1]}x{cfg['grid'][2]} workload from reset: 2 untimed + {n} timed steps, {dt:.1f} s"}
 
 
@@ -119,6 +120,17 @@ def read_traffic(kernel_names, grid):
         if kv.get("grid") == gtag and kv.get("kernel") in kernel_names:
             return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
     return None
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def calibrate_copy():
@@ -340,10 +352,12 @@ def main():
     else:
         kernel_names = [PASS_KERNELS[dominant][0], PASS_KERNELS[dominant][0] + "<false>"]
     traffic = read_traffic(kernel_names, cfg["grid"])
+    copy_gbs = firesim.copy_bandwidth(1 << 30, 20)  # achievable: streaming float4 copy of 1 GiB
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kernel_names[0], "pass": dominant, "launch_ms": round(launch_ms, 5),
-                "alg_bytes_per_launch": bytes_per_launch}
+                "alg_bytes_per_launch": bytes_per_launch,
+                "copy_GBps": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)}
 
     render = None
     if not args.no_render:

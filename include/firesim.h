@@ -234,6 +234,14 @@ int fs_write_bmp(const char* path, const uint32_t* bgra, int32_t width, int32_t 
 /* Thread-local message of the last failed call that had no handle. */
 const char* fs_last_global_error(void);
 
+/* ---- device measurement ---- */
+/* Achievable device-memory bandwidth of `device`: `reps` streaming float4
+ * copies of `bytes` (a multiple of 16, 1 MiB .. 64 GiB) between two buffers
+ * the call allocates and frees; *gbps = (read + write bytes) per second / 1e9,
+ * from HIP events around the copies. No counterpart in the reference: the
+ * achievable ceiling reported beside the nominal HBM peak. */
+int fs_copy_bandwidth(int32_t device, uint64_t bytes, int32_t reps, double* gbps);
+
 /* ---- scene lifecycle ---- */
 int fs_create(const fs_scene_desc* desc, fs_scene** out);
 /* fs_create in caller-owned device memory (256-byte aligned, at least

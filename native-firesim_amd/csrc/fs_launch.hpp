@@ -151,6 +151,8 @@ hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t s
 hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_project(DeviceScene& s, const DevParams& p, hipStream_t st);
+// bytes / 16 float4 elements from a to b (bytes a multiple of 16).
+hipError_t launch_copy4(const void* a, void* b, size_t bytes, hipStream_t st);
 hipError_t launch_light(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_render(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_tonemap(DeviceScene& s, const DevParams& p, hipStream_t st);

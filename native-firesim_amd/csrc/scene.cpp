@@ -388,6 +388,47 @@ extern "C" {
 
 int fs_abi_version(void) { return FS_ABI_VERSION; }
 
+int fs_copy_bandwidth(int32_t device, uint64_t bytes, int32_t reps, double* gbps) {
+    if (!gbps) return global_fail(FS_ERR_INVALID_ARGUMENT, "null output");
+    *gbps = 0.0;
+    if (bytes < (1ull << 20) || bytes > (64ull << 30) || bytes % 16 != 0 || reps < 1 || reps > 1000)
+        return global_fail(FS_ERR_INVALID_ARGUMENT, "bytes must be a multiple of 16 in [1 MiB, 64 GiB] and reps in [1, 1000]");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return global_fail(FS_ERR_HIP, "no HIP device");
+    if (device < 0 || device >= count) return global_fail(FS_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    e = hipSetDevice(device);
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, st);
+    if (e == hipSuccess) e = fs::launch_copy4(a, b, bytes, st);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; e == hipSuccess && r < reps; ++r) e = fs::launch_copy4(a, b, bytes, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return global_fail(e == hipErrorOutOfMemory ? FS_ERR_OUT_OF_MEMORY : FS_ERR_HIP, std::string("copy bandwidth: ") + hipGetErrorString(e));
+    }
+    if (ms <= 0.0f) return global_fail(FS_ERR_HIP, "copy bandwidth: no elapsed time");
+    *gbps = 2.0 * static_cast<double>(bytes) * reps / (static_cast<double>(ms) * 1e-3) / 1e9;
+    return FS_OK;
+}
+
 void fs_default_params(fs_params* o) {
     if (!o) return;
     std::memset(o, 0, sizeof(*o));

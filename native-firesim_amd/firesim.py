@@ -198,6 +198,16 @@ def light_dims(nx: int, ny: int, nz: int) -> tuple[int, int, int]:
     return c(nx, 18, 48), c(ny, 20, 56), c(nz, 16, 36)
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
+    """Achievable device-memory bandwidth in GB/s (read + write bytes of
+    streaming float4 copies, fs_copy_bandwidth)."""
+    lib = load_library()
+    out = c_double()
+    if lib.fs_copy_bandwidth(device, nbytes, reps, byref(out)) != 0:
+        raise FireSimError(lib.fs_last_global_error().decode())
+    return out.value
+
+
 def field_shape(field: int, nx: int, ny: int, nz: int) -> tuple:
     """Numpy shape (z-major, x fastest) of a field."""
     if field < 10 or field in (FIELD["pressure"], FIELD["divergence"]):
@@ -240,6 +250,7 @@ def load_library(path: str = LIB_PATH):
         "fs_scene_bytes": (c_int32, [P(SceneDesc), P(c_uint64)]),
         "fs_write_bmp": (c_int32, [c_char_p, P(c_uint32), c_int32, c_int32]),
         "fs_last_global_error": (c_char_p, []),
+        "fs_copy_bandwidth": (c_int32, [c_int32, c_uint64, c_int32, P(c_double)]),
         "fs_create": (c_int32, [P(SceneDesc), P(c_void_p)]),
         "fs_destroy": (c_int32, [c_void_p]),
         "fs_reset": (c_int32, [c_void_p]),

@@ -94,3 +94,11 @@ def test_bmp_matches_reference_layout(tmp_path):
     assert np.array_equal(np.frombuffer(data[54:], dtype="<u4"), pix)
     assert LIB.fs_write_bmp(path.encode(), None, w, h) == 1
     assert LIB.fs_write_bmp(b"/nonexistent-dir/x.bmp", pix.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), w, h) == 6
+
+
+def test_copy_bandwidth_rejects_bad_sizes_before_touching_a_device():
+    out = ctypes.c_double(-1.0)
+    for nbytes, reps in ((0, 1), ((1 << 20) + 8, 1), (1 << 20, 0), (1 << 20, 1001), (65 << 30, 1)):
+        assert LIB.fs_copy_bandwidth(0, nbytes, reps, ctypes.byref(out)) == 1, (nbytes, reps)
+        assert out.value == 0.0
+    assert LIB.fs_copy_bandwidth(0, 1 << 20, 1, None) == 1

@@ -52,3 +52,11 @@ def test_autotune_restores_pressure_and_picks_a_kernel(hip_lib):
     assert np.array_equal(s.read("divergence"), d)
     assert firesim.SOR_KERNELS[s.sor_kernel()] == rep["chosen"]
     s.close()
+
+
+def test_copy_bandwidth_is_a_plausible_hbm_rate(hip_lib):
+    """Streaming copy of 256 MiB: read + write bytes per second below the
+    nominal 8 TB/s HBM peak and above a tenth of it."""
+    gbps = firesim.copy_bandwidth(256 << 20, 5)
+    print("copy GB/s", gbps, flush=True)
+    assert 800.0 < gbps < 8000.0
