@@ -146,7 +146,8 @@ dim3 sor_grid(int sorKernel, int nx, int ny, int nz);
 
 hipError_t launch_reset(const DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_advance_clock(DeviceScene& s, const DevParams& p, hipStream_t st);
-hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st);
+// advanceClock: also advance the device clock by one step (launch_advance_clock).
+hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st, bool advanceClock = false);
 hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st);

@@ -263,11 +263,13 @@ int run_stage(fs_scene* s, int stage, const DevParams& p) {
     return e == hipSuccess ? FS_OK : s->hip_fail(what, e);
 }
 
-// One step's device work: advance the device clock, then the six passes.
+// One step's device work: advance the device clock, then the six passes (the
+// clock advances inside the velocity-advection launch, which does not read it).
 int enqueue_step(fs_scene* s, const DevParams& p) {
-    hipError_t e = fs::launch_advance_clock(s->ds, p, s->stream);
-    if (e != hipSuccess) return s->hip_fail("advance clock", e);
-    for (int stage = 0; stage < FS_STAGE_COUNT; ++stage)
+    static_assert(FS_STAGE_ADVECT_VELOCITY == 0, "velocity advection is the first pass");
+    const hipError_t e = fs::launch_advect_velocity(s->ds, p, s->stream, true);
+    if (e != hipSuccess) return s->hip_fail("advect velocity", e);
+    for (int stage = FS_STAGE_ADVECT_VELOCITY + 1; stage < FS_STAGE_COUNT; ++stage)
         if (int st = run_stage(s, stage, p)) return st;
     return FS_OK;
 }

@@ -113,9 +113,16 @@ __device__ __forceinline__ int3 cell_of(uint3 blk) {
                      static_cast<int>(blk.z * blockDim.z + threadIdx.z));
 }
 
+// With a non-null `clock`, one thread also advances the device clock by one
+// step (as k_advance_clock does): this pass does not read the clock, and every
+// later pass of the step sees the advanced value.
 __global__ __launch_bounds__(256) void k_advect_velocity(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
                                                          const float* __restrict__ U, const float* __restrict__ V,
-                                                         const float* __restrict__ W, DevParams p) {
+                                                         const float* __restrict__ W, DevParams p, float* __restrict__ clock) {
+    if (clock != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        clock[0] += p.dt;
+        clock[1] += p.dt;
+    }
     const int3 c = cell_of(xcd_block());
     advect_velocity_at(uo, vo, wo, U, V, W, p, c.x, c.y, c.z);
 }
@@ -923,9 +930,9 @@ hipError_t launch_advance_clock(DeviceScene& s, const DevParams& p, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st) {
+hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st, bool advanceClock) {
     const dim3 g = cell_grid(s.nx + 1, s.ny + 1, s.nz + 1), b = cell_block();
-    k_advect_velocity<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, p);
+    k_advect_velocity<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, p, advanceClock ? s.clock : nullptr);
     std::swap(s.u, s.uN);
     std::swap(s.v, s.vN);
     std::swap(s.w, s.wN);
