@@ -148,8 +148,10 @@ hipError_t launch_reset(const DeviceScene& s, const DevParams& p, hipStream_t st
 hipError_t launch_advance_clock(DeviceScene& s, const DevParams& p, hipStream_t st);
 // advanceClock: also advance the device clock by one step (launch_advance_clock).
 hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st, bool advanceClock = false);
-hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st);
-hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st);
+// finalize = false leaves the heat-release sum of the react partials to the
+// next launch_force with finalizeRelease = true (one launch fewer per step).
+hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalize = true);
+hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalizeRelease = false);
 hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st);
 hipError_t launch_project(DeviceScene& s, const DevParams& p, hipStream_t st);
 // bytes / 16 float4 elements from a to b (bytes a multiple of 16).

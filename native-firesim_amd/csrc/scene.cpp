@@ -248,13 +248,15 @@ int set_device(fs_scene* s) {
     return e == hipSuccess ? FS_OK : s->hip_fail("hipSetDevice", e);
 }
 
-int run_stage(fs_scene* s, int stage, const DevParams& p) {
+// fused: the step's form, where the force launch also sums the react
+// partials (a pass run alone, or timed alone, finishes its own work).
+int run_stage(fs_scene* s, int stage, const DevParams& p, bool fused = false) {
     hipError_t e = hipSuccess;
     const char* what = "";
     switch (stage) {
         case FS_STAGE_ADVECT_VELOCITY: e = fs::launch_advect_velocity(s->ds, p, s->stream); what = "advect velocity"; break;
-        case FS_STAGE_ADVECT_REACT: e = fs::launch_advect_react(s->ds, p, s->stream); what = "advect/react"; break;
-        case FS_STAGE_FORCE: e = fs::launch_force(s->ds, p, s->stream); what = "force/boundary/divergence"; break;
+        case FS_STAGE_ADVECT_REACT: e = fs::launch_advect_react(s->ds, p, s->stream, !fused); what = "advect/react"; break;
+        case FS_STAGE_FORCE: e = fs::launch_force(s->ds, p, s->stream, fused); what = "force/boundary/divergence"; break;
         case FS_STAGE_PRESSURE: e = fs::launch_pressure(s->ds, p, s->stream); what = "pressure SOR"; break;
         case FS_STAGE_PROJECT: e = fs::launch_project(s->ds, p, s->stream); what = "pressure subtract"; break;
         case FS_STAGE_LIGHT: e = fs::launch_light(s->ds, p, s->stream); what = "scene lighting"; break;
@@ -269,8 +271,9 @@ int enqueue_step(fs_scene* s, const DevParams& p) {
     static_assert(FS_STAGE_ADVECT_VELOCITY == 0, "velocity advection is the first pass");
     const hipError_t e = fs::launch_advect_velocity(s->ds, p, s->stream, true);
     if (e != hipSuccess) return s->hip_fail("advect velocity", e);
+    static_assert(FS_STAGE_FORCE == FS_STAGE_ADVECT_REACT + 1, "force follows advect-react");
     for (int stage = FS_STAGE_ADVECT_VELOCITY + 1; stage < FS_STAGE_COUNT; ++stage)
-        if (int st = run_stage(s, stage, p)) return st;
+        if (int st = run_stage(s, stage, p, true)) return st;
     return FS_OK;
 }
 

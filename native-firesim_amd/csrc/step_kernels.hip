@@ -332,11 +332,12 @@ __global__ void k_advance_clock(float* clock, float dt) {
 
 // Thread t's sum of p[t], p[t + 256], p[t + 512], ... added in that order,
 // with eight loads in flight (a loop of one load and one add waits a full
-// memory latency per element).
-__device__ __forceinline__ double strided_sum_256(const double* __restrict__ p, size_t n) {
+// memory latency per element). t is the thread's linear id in a 256-thread
+// workgroup.
+__device__ __forceinline__ double strided_sum_256(const double* __restrict__ p, size_t n, unsigned t) {
     constexpr size_t S = 256;
     double s = 0.0;
-    size_t i = threadIdx.x;
+    size_t i = t;
     for (; i + 7 * S < n; i += 8 * S) {
         double v[8];
 #pragma unroll
@@ -350,7 +351,7 @@ __device__ __forceinline__ double strided_sum_256(const double* __restrict__ p, 
 
 // Sums the react partials in a fixed order: acc[0] = step total, acc[1] += it.
 __global__ __launch_bounds__(256) void k_finalize_release(const double* __restrict__ partial, size_t n, double* __restrict__ acc) {
-    const double t = block_sum_256(strided_sum_256(partial, n));
+    const double t = block_sum_256(strided_sum_256(partial, n, threadIdx.x));
     if (threadIdx.x == 0) {
         acc[0] = t;
         acc[1] += t;
@@ -404,10 +405,22 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
     st_at(cw, idx, p.dt * (p.windZ * windProfile + (gradX * 1.36f + noiseB * (0.20f + upperDrag * 0.16f) + curlZ * vortexGain) * curlGain));
 }
 
+// With a non-null `partial`, workgroup (0, 0, 0) first sums the react
+// partials as k_finalize_release does (same order, same result), hidden
+// behind the other workgroups of this launch.
 template <bool B>
 __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, Scalars<B> S,
                                                        const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                                                       DevParams p) {
+                                                       DevParams p, const double* __restrict__ partial, size_t npartial,
+                                                       double* __restrict__ acc) {
+    if (partial != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) {
+        const unsigned t = threadIdx.x + blockDim.x * threadIdx.y;
+        const double sum = block_sum_256(strided_sum_256(partial, npartial, t));
+        if (t == 0) {
+            acc[0] = sum;
+            acc[1] += sum;
+        }
+    }
     const int3 c = cell_of(xcd_block());
     force_contrib_at<B>(cu, cv, cw, S, U, V, W, p, c.x, c.y, c.z);
 }
@@ -854,7 +867,7 @@ __global__ __launch_bounds__(256) void kb_finalize_release(const BatchTable* __r
     for (int s = blockIdx.x; s < T->count; s += gridDim.x) {
         const BatchScene& S = T->scene[s];
         const size_t n = static_cast<size_t>(S.cellG[0]) * S.cellG[1] * S.cellG[2];
-        const double t = block_sum_256_loop(strided_sum_256(S.partials, n));
+        const double t = block_sum_256_loop(strided_sum_256(S.partials, n, threadIdx.x));
         if (threadIdx.x == 0) {
             S.acc[0] = t;
             S.acc[1] += t;
@@ -939,19 +952,21 @@ hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st) {
+hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalize) {
     const dim3 g = cell_grid(s.nx, s.ny, s.nz), b = cell_block();
     if (s.bf16) k_advect_react<true><<<g, b, 0, st>>>(scal<true>(s.sNext), scal<true>(s.s), s.u, s.v, s.w, p, s.partials);
     else k_advect_react<false><<<g, b, 0, st>>>(scal<false>(s.sNext), scal<false>(s.s), s.u, s.v, s.w, p, s.partials);
-    k_finalize_release<<<1, 256, 0, st>>>(s.partials, react_partial_count(s.nx, s.ny, s.nz), s.acc);
+    if (finalize) k_finalize_release<<<1, 256, 0, st>>>(s.partials, react_partial_count(s.nx, s.ny, s.nz), s.acc);
     for (int i = 0; i < NSCALAR; ++i) std::swap(s.s[i], s.sNext[i]);
     return hipGetLastError();
 }
 
-hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st) {
+hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalizeRelease) {
     const dim3 g = cell_grid(s.nx, s.ny, s.nz), b = cell_block();
-    if (s.bf16) k_force_contrib<true><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, scal<true>(s.s), s.u, s.v, s.w, p);
-    else k_force_contrib<false><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, scal<false>(s.s), s.u, s.v, s.w, p);
+    const double* part = finalizeRelease ? s.partials : nullptr;
+    const size_t np = react_partial_count(s.nx, s.ny, s.nz);
+    if (s.bf16) k_force_contrib<true><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, scal<true>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
+    else k_force_contrib<false><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, scal<false>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
     k_force_apply<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, s.cu, s.cv, s.cw, s.d, s.p, p);
     std::swap(s.u, s.uN);
     std::swap(s.v, s.vN);
