@@ -776,8 +776,10 @@ static Scalars<B> scal(void* const* f) {
     return s;
 }
 
-dim3 light_block() { return dim3(8, 8, 4); }
-dim3 light_grid(int lnx, int lny, int lnz) { return dim3((lnx + 7) / 8, (lny + 7) / 8, (lnz + 3) / 4); }
+// One wavefront per 8x8 light-cell tile: the quarter-resolution volume has few
+// cells (32^3 at 128^3), so smaller workgroups spread it over every CU.
+dim3 light_block() { return dim3(8, 8, 1); }
+dim3 light_grid(int lnx, int lny, int lnz) { return dim3((lnx + 7) / 8, (lny + 7) / 8, lnz); }
 
 hipError_t launch_light(DeviceScene& s, const DevParams& p, hipStream_t st) {
     const dim3 b = light_block();
