@@ -59,11 +59,14 @@ __device__ __forceinline__ F3 vel_from_axes(const float* __restrict__ U, const f
               tri_plain(W, tri_of(xC, yC, zF, p.nx, p.ny)));
 }
 
-// Thread (x, y, z) of a face grid of (nx+1, ny+1, nz+1) writes u, v and w faces.
+// Thread (x, y, z) of a face grid of (nx, ny+1, nz+1) writes the u, v and w
+// faces at (x, y, z); the thread at x = nx - 1 also writes the wall u face at
+// x = nx (0), so no wavefront is spent on that one column.
 __device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
                                                    const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                    const DevParams& p, int x, int y, int z) {
-    if (x > p.nx || y > p.ny || z > p.nz) return;
+    if (x >= p.nx || y > p.ny || z > p.nz) return;
+    if (x == p.nx - 1 && y < p.ny && z < p.nz) st_at(uo, uidx(p, p.nx, y, z), 0.0f);
     // backtrace positions in double (see tri_setup); sampled values and outputs in float
     const double dnx = p.nx, dny = p.ny, dnz = p.nz;
     const double k = static_cast<double>(p.dt) * static_cast<double>(kAdvectionScale);
@@ -636,9 +639,11 @@ __global__ __launch_bounds__((T + 2 * K) * (T + 2 * K)) void k_sor_tb(const floa
 }
 
 // ----------------------------------------------------------- K13-K14 ----
+// Face grid of (nx, ny+1, nz+1), as advect_velocity_at (x = nx - 1 also zeroes the wall u face at x = nx).
 __device__ __forceinline__ void project_at(float* __restrict__ U, float* __restrict__ V, float* __restrict__ W, const float* __restrict__ P,
                                            const DevParams& p, int x, int y, int z) {
-    if (x > p.nx || y > p.ny || z > p.nz) return;
+    if (x >= p.nx || y > p.ny || z > p.nz) return;
+    if (x == p.nx - 1 && y < p.ny && z < p.nz) st_at(U, uidx(p, p.nx, y, z), 0.0f);
     if (y < p.ny && z < p.nz) {
         const Idx f = uidx(p, x, y, z);
         if (x == 0 || x == p.nx) st_at(U, f, 0.0f);
@@ -944,7 +949,7 @@ hipError_t launch_advance_clock(DeviceScene& s, const DevParams& p, hipStream_t 
 }
 
 hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_t st, bool advanceClock) {
-    const dim3 g = cell_grid(s.nx + 1, s.ny + 1, s.nz + 1), b = cell_block();
+    const dim3 g = cell_grid(s.nx, s.ny + 1, s.nz + 1), b = cell_block();
     k_advect_velocity<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, p, advanceClock ? s.clock : nullptr);
     std::swap(s.u, s.uN);
     std::swap(s.v, s.vN);
@@ -1065,7 +1070,7 @@ hipError_t launch_copy4(const void* a, void* b, size_t bytes, hipStream_t st) {
 }
 
 hipError_t launch_project(DeviceScene& s, const DevParams& p, hipStream_t st) {
-    const dim3 g = cell_grid(s.nx + 1, s.ny + 1, s.nz + 1), b = cell_block();
+    const dim3 g = cell_grid(s.nx, s.ny + 1, s.nz + 1), b = cell_block();
     k_project<<<g, b, 0, st>>>(s.u, s.v, s.w, s.p, p);
     return hipGetLastError();
 }
