@@ -847,7 +847,9 @@ __global__ __launch_bounds__(256) void kb_advance_clock(const BatchTable* __rest
     }
 }
 
-__global__ __launch_bounds__(256) void kb_advect_velocity(const BatchTable* __restrict__ T) {
+// The persistent loop lifts this kernel to 86 VGPRs (five wavefronts per
+// SIMD); held to eight wavefronts it needs 63 and spills nothing.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void kb_advect_velocity(const BatchTable* __restrict__ T) {
     batch_blocks(T->faceStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
         const int3 c = cell_of(block_of(local, S.faceG));
@@ -881,7 +883,8 @@ __global__ __launch_bounds__(256) void kb_finalize_release(const BatchTable* __r
 }
 
 template <bool B>
-__global__ __launch_bounds__(256) void kb_force_contrib(const BatchTable* __restrict__ T) {
+// Held to eight wavefronts per SIMD as kb_advect_velocity (67 -> 64 VGPRs, no spills).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void kb_force_contrib(const BatchTable* __restrict__ T) {
     batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
         const int3 c = cell_of(block_of(local, S.cellG));
