@@ -882,8 +882,8 @@ __global__ __launch_bounds__(256) void kb_finalize_release(const BatchTable* __r
     }
 }
 
-template <bool B>
 // Held to eight wavefronts per SIMD as kb_advect_velocity (67 -> 64 VGPRs, no spills).
+template <bool B>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void kb_force_contrib(const BatchTable* __restrict__ T) {
     batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
