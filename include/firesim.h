@@ -205,7 +205,8 @@ enum fs_sor_kernel {
     FS_SOR_TB_T24_Z6_K4 = 8,    /* 24x24x6 tiles, for small grids */
     FS_SOR_TB_T26_Z13_K3 = 9,   /* 26x26x13 tiles, 3 half-sweeps per launch (5 tiles span 128 cells) */
     FS_SOR_TB_T26_Z9_K3 = 10,   /* 26x26x9 tiles, 3 half-sweeps per launch, two workgroups per CU */
-    FS_SOR_KERNEL_COUNT = 11
+    FS_SOR_TB_T22_Z8_K5 = 11,   /* 22x22x8 tiles, 5 half-sweeps per launch: fewest launches, for small grids */
+    FS_SOR_KERNEL_COUNT = 12
 };
 
 /* Launch-shape choices of a scene (picked by the autotuner). */

@@ -541,7 +541,8 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
 // 16^3 tiles (55 KB of LDS, two workgroups per CU) suit small grids, 24x24
 // tiles with a 32x32 loaded face (1024 threads, one or two per CU) larger ones.
 // The 26x26 tiles run K = 3 on the same 32x32 face: less halo per cell than
-// 24x24 and five of them span 128 cells (six 24-wide tiles span 144).
+// 24x24 and five of them span 128 cells (six 24-wide tiles span 144). The
+// 22x22 tiles run K = 5: the fewest launches, for small, launch-bound grids.
 // Unroll depth of the sweep loop: 4 for deep tiles (one workgroup per CU,
 // register room to spare), 2 where two workgroups share a CU.
 constexpr int sor_unroll(int tz) { return tz >= 24 ? 4 : 2; }
@@ -999,6 +1000,7 @@ static SorShape sor_shape(int sorKernel) {
         case 8: return {24, 6, 4};
         case 9: return {26, 13, 3};
         case 10: return {26, 9, 3};
+        case 11: return {22, 8, 5};
         default: return {0, 0, 1};  // plain half-sweep kernel
     }
 }
@@ -1032,6 +1034,7 @@ hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
         case 8: launch_sor_tb<24, 4, 6>(s, p, st); break;
         case 9: launch_sor_tb<26, 3, 13>(s, p, st); break;
         case 10: launch_sor_tb<26, 3, 9>(s, p, st); break;
+        case 11: launch_sor_tb<22, 5, 8>(s, p, st); break;
         default: {
             const dim3 b = cell_block();
             const dim3 g = sor_grid(0, s.nx, s.ny, s.nz);
@@ -1131,6 +1134,7 @@ hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sor
         case 8: launch_batch_sor_tb<24, 4, 6>(table, sorLaunches, grid, st); break;
         case 9: launch_batch_sor_tb<26, 3, 13>(table, sorLaunches, grid, st); break;
         case 10: launch_batch_sor_tb<26, 3, 9>(table, sorLaunches, grid, st); break;
+        case 11: launch_batch_sor_tb<22, 5, 8>(table, sorLaunches, grid, st); break;
         default:
             for (int j = 0; j < sorLaunches; ++j) kb_sor_half<<<grid, cell_block(), 0, st>>>(table, j);
     }

@@ -32,7 +32,8 @@ STAGE_NAMES = ["advect_velocity", "advect_react", "force", "pressure", "project"
 
 STORAGE_FP32, STORAGE_BF16 = 0, 1
 SOR_KERNELS = ["half_sweep", "tb_t16_k4", "tb_t24_k4", "tb_t24_z8_k4", "tb_t24_z16_k4", "tb_t24_z28_k4", "tb_t24_z10_k4", "tb_t24_z12_k4",
-               "tb_t24_z6_k4", "tb_t26_z13_k3", "tb_t26_z9_k3"]
+               "tb_t24_z6_k4", "tb_t26_z13_k3", "tb_t26_z9_k3",
+               "tb_t22_z8_k5"]
 SCENE_ROOM_TRAY, SCENE_CAMPFIRE, SCENE_GAS_BURNER, SCENE_EMITTERS_ONLY = 0, 1, 2, 3
 MAX_EMITTERS = 16
 

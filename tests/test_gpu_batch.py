@@ -37,7 +37,7 @@ def assert_same(a, b, what):
     assert a.get_time() == b.get_time(), what
 
 
-@pytest.mark.parametrize("sor_kernel", ["half_sweep", "tb_t16_k4", "tb_t24_k4", "tb_t24_z8_k4", "tb_t24_z10_k4", "tb_t24_z6_k4", "tb_t26_z13_k3"])
+@pytest.mark.parametrize("sor_kernel", ["half_sweep", "tb_t16_k4", "tb_t24_k4", "tb_t24_z8_k4", "tb_t24_z10_k4", "tb_t24_z6_k4", "tb_t26_z13_k3", "tb_t22_z8_k5"])
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_equals_alone(hip_lib, sor_kernel, use_graph):
     alone = [make(g, 11 + i) for i, g in enumerate(GRIDS)]
