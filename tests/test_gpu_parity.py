@@ -211,3 +211,21 @@ def test_bf16_storage_error(hip_lib):
     ms, mo = s.metrics(), o.metrics()
     assert ms["heat_release_total"] == pytest.approx(mo["heat_release_total"], rel=0.05)
     assert errs["heat"] < 0.05 and errs["u"] < 0.05
+
+
+@pytest.mark.parametrize("grid", [(256, 12, 8), (48, 20, 16)])
+def test_force_parity_across_noise_sharing(hip_lib, grid):
+    """Force pass against the oracle where a wavefront's 64 cells share the
+    lattice values of every noise octave (256 wide) and where the upper
+    octaves span more than 64 lattice points and use per-lane hashes (48 wide)."""
+    params = compare.emitter_params(wind_x=0.3, wind_z=-0.2, turbulence=0.6)
+    em = [(grid[0] / 2.0, 3.0, grid[2] / 2.0, 3.0, 1.0, 1500.0)]
+    state, t, rt = compare.warm_state(*grid, params, em, steps=10)
+    s, o = _pair(params, em, grid=grid)
+    compare.load_both(s, o, state, t, rt)
+    s.run_stage(firesim.STAGE_FORCE)
+    o.run_stage(firesim.STAGE_FORCE)
+    errs = compare.compare_fields(s, o, compare.STAGE_OUTPUTS[firesim.STAGE_FORCE])
+    print(grid, errs, flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= TOL}
+    assert not bad, f"force pass exceeds {TOL} on {grid}: {bad}"
