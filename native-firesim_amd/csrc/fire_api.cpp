@@ -208,7 +208,13 @@ bool fireCudaInitialize(int frameWidth, int frameHeight, int gridWidth, int grid
     g_frameIndex = 0;
     g_haveApplied = false;
     std::snprintf(g_lastError, sizeof(g_lastError), "No error.");
-    return apply(FireSettings{});
+    if (!apply(FireSettings{})) return false;
+    // The fastest SOR tile for this grid (every tile gives the same result):
+    // one timed pressure pass per tile on the freshly reset fields, which the
+    // autotuner leaves as they were.
+    fs_autotune_report report{};
+    if (fs_autotune(g_scene, 1, &report) != FS_OK) return fail_fs("SOR autotune");
+    return true;
 }
 
 bool fireCudaStepAndRender(std::uint32_t* bgraPixels, const FireSettings& settings) {
