@@ -398,7 +398,7 @@ typedef struct fs_request_status {
     int32_t state;              /* enum fs_request_state */
     int32_t steps_done;         /* steps whose completion has been observed */
     int32_t frames;             /* frames rendered */
-    int32_t reserved;
+    int32_t frame_w, frame_h;   /* frame size of the request (fs_server_frame writes frame_w*frame_h pixels) */
     double arrival_s, admitted_s, finished_s;
     double step_ms_p50, step_ms_p99, step_ms_max;
 } fs_request_status;
@@ -421,12 +421,18 @@ int fs_server_create(const fs_server_desc* desc, fs_server** out);
 /* Destroys every request's scene and frees the server. */
 int fs_server_destroy(fs_server* server);
 int fs_server_submit(fs_server* server, const fs_request* request, uint64_t* id);
+/* Every fs_server_* call holds the server's lock while it touches server
+ * state: requests may be submitted and status read from other threads while
+ * one thread runs fs_server_run or fs_server_tick. fs_server_destroy must not
+ * overlap any other call on the server. */
 /* One scheduling round; *active receives the number of requests stepped. */
 int fs_server_tick(fs_server* server, int32_t* active);
 /* Rounds until every submitted request is done (sleeps until the next arrival when idle). */
 int fs_server_run(fs_server* server);
 int fs_server_status(fs_server* server, uint64_t id, fs_request_status* out);
-/* Latest frame of a request (frame_w*frame_h BGRA pixels); FS_ERR_INVALID_ARGUMENT if none yet. */
+/* Latest frame of a request: frame_w*frame_h BGRA pixels of the request's
+ * descriptor (reported by fs_server_status), so bgra must hold that many;
+ * FS_ERR_INVALID_ARGUMENT if none yet. */
 int fs_server_frame(fs_server* server, uint64_t id, uint32_t* bgra, int32_t* frame_step);
 /* Hands a finished request's scene (kept with keep_scene) to the caller, who then owns it. */
 int fs_server_take_scene(fs_server* server, uint64_t id, fs_scene** out);

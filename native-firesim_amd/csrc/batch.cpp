@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -33,7 +34,11 @@ struct fs_batch {
     BatchTable* hostTable[2] = {nullptr, nullptr};  // pinned ring
     hipEvent_t hostFree[2] = {nullptr, nullptr};    // recorded after the copy out of hostTable[i]
     int ring = 0;
-    std::map<int, hipGraphExec_t> graphs;  // captured steps, by pressure launch count
+    // Captured steps. A graph's kernels are fixed at capture by the SOR kernel,
+    // the storage mode (the FP32 or BF16 passes) and the pressure launch count,
+    // so all three key it; the table it reads is rewritten before every launch.
+    std::map<std::tuple<int, bool, int>, hipGraphExec_t> graphs;
+    GraphGraveyard graveyard;  // dropped graphs whose last launch may still run
     hipEvent_t joinEvent = nullptr;
     std::string lastError = "no error";
 
@@ -120,9 +125,12 @@ int build_table(const fs_batch* b, int sorKernel, BatchTable* t) {
 
 size_t table_bytes(int count) { return fs::batch_table_bytes(std::max(count, 1)); }
 
+// Graphs may have launches queued or running on the batch stream: they are
+// destroyed once the stream has passed them.
 void drop_batch_graphs(fs_batch* b) {
-    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto& kv : b->graphs) b->graveyard.retire(kv.second, b->stream);
     b->graphs.clear();
+    b->graveyard.reap(false);
 }
 
 // Host bookkeeping of one completed step, the same as fs_step's.
@@ -140,7 +148,8 @@ void advance_members(fs_batch* b) {
 }
 
 int graph_for(fs_batch* b, int launches, hipGraphExec_t* out) {
-    auto it = b->graphs.find(launches);
+    const auto key = std::make_tuple(b->sorKernel, b->bf16, launches);
+    auto it = b->graphs.find(key);
     if (it != b->graphs.end()) {
         *out = it->second;
         return FS_OK;
@@ -158,7 +167,7 @@ int graph_for(fs_batch* b, int launches, hipGraphExec_t* out) {
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return b->hip_fail("hipGraphInstantiate", e);
-    b->graphs[launches] = exec;
+    b->graphs[key] = exec;
     *out = exec;
     return FS_OK;
 }
@@ -193,7 +202,9 @@ int set_device(fs_batch* b) {
 }
 
 void release(fs_batch* b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_batch_graphs(b);
+    b->graveyard.reap(true);
     for (int i = 0; i < 2; ++i) {
         if (b->hostFree[i]) (void)hipEventDestroy(b->hostFree[i]);
         if (b->hostTable[i]) (void)hipHostFree(b->hostTable[i]);
@@ -293,12 +304,10 @@ int fs_batch_set_tuning(fs_batch* b, const fs_tuning* t) {
     if (!b) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null batch");
     if (!t) return b->fail(FS_ERR_INVALID_ARGUMENT, "null tuning");
     if (t->sor_kernel < 0 || t->sor_kernel >= FS_SOR_KERNEL_COUNT) return b->fail(FS_ERR_INVALID_ARGUMENT, "unknown SOR kernel");
-    if (t->sor_kernel != b->sorKernel) {
-        // members whose last batched step used the other kernel's buffer order
-        // are unaffected: the table is rebuilt from their state every step
-        drop_batch_graphs(b);
-        b->sorKernel = t->sor_kernel;
-    }
+    // Members whose last batched step used the other kernel's buffer order are
+    // unaffected: the table is rebuilt from their state every step. Graphs are
+    // keyed by kernel, so those of the other kernel stay valid.
+    b->sorKernel = t->sor_kernel;
     b->useGraph = t->use_graph != 0;
     return FS_OK;
 }
@@ -352,10 +361,7 @@ int fs_batch_autotune(fs_batch* b, int32_t reps, fs_autotune_report* out) {
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     if (e != hipSuccess) return b->hip_fail("batch autotune", e);
-    if (best != b->sorKernel) {
-        drop_batch_graphs(b);
-        b->sorKernel = best;
-    }
+    b->sorKernel = best;
     rep.chosen = best;
     if (out) *out = rep;
     return FS_OK;

@@ -208,12 +208,21 @@ bool fireCudaInitialize(int frameWidth, int frameHeight, int gridWidth, int grid
     g_frameIndex = 0;
     g_haveApplied = false;
     std::snprintf(g_lastError, sizeof(g_lastError), "No error.");
-    if (!apply(FireSettings{})) return false;
+    // A failed initialization frees everything it created, as the reference's
+    // does (:2981): later calls then report "not initialized".
+    if (!apply(FireSettings{})) {
+        fireCudaShutdown();
+        return false;
+    }
     // The fastest SOR tile for this grid (every tile gives the same result):
     // one timed pressure pass per tile on the freshly reset fields, which the
     // autotuner leaves as they were.
     fs_autotune_report report{};
-    if (fs_autotune(g_scene, 1, &report) != FS_OK) return fail_fs("SOR autotune");
+    if (fs_autotune(g_scene, 1, &report) != FS_OK) {
+        fail_fs("SOR autotune");
+        fireCudaShutdown();
+        return false;
+    }
     return true;
 }
 

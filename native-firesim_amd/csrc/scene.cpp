@@ -277,12 +277,15 @@ int enqueue_step(fs_scene* s, const DevParams& p) {
     return FS_OK;
 }
 
+// The graphs may have launches queued or running on the scene stream: they go
+// to the graveyard and are destroyed once the stream has passed them.
 void drop_graphs(fs_scene* s) {
     for (auto& g : s->exec) {
-        if (g) (void)hipGraphExecDestroy(g);
+        s->graveyard.retire(g, s->stream);
         g = nullptr;
     }
     s->graphsValid = false;
+    s->graveyard.reap(false);
 }
 
 bool same_buffers(const DeviceScene& a, const DeviceScene& b) {
@@ -582,6 +585,7 @@ int fs_destroy(fs_scene* s) {
     if (s->batch) (void)fs_batch_remove(s->batch, s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     drop_graphs(s);
+    s->graveyard.reap(true);
     if (s->ownStream) (void)hipStreamDestroy(s->ownStream);
     if (s->pool && s->ownsPool) (void)hipFree(s->pool);
     delete s;

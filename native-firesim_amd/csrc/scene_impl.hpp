@@ -15,6 +15,50 @@ using fs::DevParams;
 
 struct fs_batch;
 
+// Executable graphs that may still be running. A graph dropped while a launch
+// of it is queued or running on a stream is not destroyed at once: retire()
+// records an event on that stream behind the launch, and the graph is
+// destroyed by a later reap() once the event has completed (reap(true) waits
+// for it). So no hipGraphExecDestroy ever races a launch in flight.
+class GraphGraveyard {
+public:
+    void retire(hipGraphExec_t g, hipStream_t st) {
+        if (!g) return;
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, st) != hipSuccess) {
+            // no event: fall back to waiting for the stream before destroying
+            (void)hipGetLastError();
+            if (ev) (void)hipEventDestroy(ev);
+            (void)hipStreamSynchronize(st);
+            (void)hipGraphExecDestroy(g);
+            return;
+        }
+        dead_.push_back({g, ev});
+    }
+    void reap(bool wait) {
+        size_t keep = 0;
+        for (size_t i = 0; i < dead_.size(); ++i) {
+            Entry& e = dead_[i];
+            const hipError_t q = wait ? hipEventSynchronize(e.done) : hipEventQuery(e.done);
+            if (q == hipErrorNotReady) {
+                dead_[keep++] = e;
+                continue;
+            }
+            (void)hipGraphExecDestroy(e.exec);
+            (void)hipEventDestroy(e.done);
+        }
+        dead_.resize(keep);
+    }
+    size_t pending() const { return dead_.size(); }
+
+private:
+    struct Entry {
+        hipGraphExec_t exec;
+        hipEvent_t done;
+    };
+    std::vector<Entry> dead_;
+};
+
 struct Dims {
     int nx, ny, nz, lnx, lny, lnz, fw, fh;
     bool bf16;
@@ -50,6 +94,7 @@ struct fs_scene {
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     DeviceScene state[2];
     int parity = 0;  // graph to run next
+    GraphGraveyard graveyard;  // dropped step graphs whose last launch may still run
 
     int fail(int status, const std::string& msg) {
         lastError = msg;

@@ -74,7 +74,10 @@ double pct(std::vector<double> v, double q) {
 struct fs_server {
     fs_server_desc desc{};
     fs_batch* batch = nullptr;
-    std::mutex mu;  // guards submissions against the scheduling thread
+    // Every entry point holds mu while it reads or changes server state, so
+    // requests may be submitted (and status read) from other threads while one
+    // thread runs the scheduling loop. fs_server_run releases it while it sleeps.
+    std::mutex mu;
     std::map<uint64_t, Request> reqs;
     std::multimap<double, uint64_t> pending;  // by arrival
     std::deque<uint64_t> waiting;             // arrived, FIFO
@@ -368,21 +371,22 @@ int fs_server_submit(fs_server* sv, const fs_request* r, uint64_t* id) {
     return FS_OK;
 }
 
-int fs_server_tick(fs_server* sv, int32_t* activeOut) {
-    if (!sv) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null server");
+}  // extern "C"
+
+namespace {
+
+// One scheduling round; the caller holds sv->mu.
+int tick_locked(fs_server* sv, int32_t* activeOut) {
     if (activeOut) *activeOut = 0;
     if (hipSetDevice(sv->desc.device) != hipSuccess) return sv->fail(FS_ERR_HIP, "hipSetDevice");
     if (int st = drain(sv, 2)) return st;
     const double t = sv->now();
-    {
-        std::lock_guard<std::mutex> lock(sv->mu);
-        while (!sv->pending.empty() && sv->pending.begin()->first <= t) {
-            const uint64_t id = sv->pending.begin()->second;
-            sv->pending.erase(sv->pending.begin());
-            sv->reqs[id].state = FS_REQ_WAITING;
-            if (sv->firstArrival < 0.0) sv->firstArrival = sv->reqs[id].arrival;
-            sv->waiting.push_back(id);
-        }
+    while (!sv->pending.empty() && sv->pending.begin()->first <= t) {
+        const uint64_t id = sv->pending.begin()->second;
+        sv->pending.erase(sv->pending.begin());
+        sv->reqs[id].state = FS_REQ_WAITING;
+        if (sv->firstArrival < 0.0) sv->firstArrival = sv->reqs[id].arrival;
+        sv->waiting.push_back(id);
     }
     while (!sv->waiting.empty() && admit(sv, sv->reqs[sv->waiting.front()], t)) sv->waiting.pop_front();
     if (sv->active.empty()) return FS_OK;
@@ -430,33 +434,44 @@ int fs_server_tick(fs_server* sv, int32_t* activeOut) {
     return FS_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int fs_server_tick(fs_server* sv, int32_t* activeOut) {
+    if (!sv) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null server");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    return tick_locked(sv, activeOut);
+}
+
 int fs_server_run(fs_server* sv) {
     if (!sv) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null server");
     for (;;) {
-        int32_t n = 0;
-        if (int st = fs_server_tick(sv, &n)) return st;
-        if (n > 0) continue;
-        if (!sv->inflight.empty()) {
-            if (int st = drain(sv, 1)) return st;  // wait for the oldest round
-            continue;
-        }
-        if (all_done(sv)) return FS_OK;
-        double next = -1.0;
+        double wait = 0.0;
         {
             std::lock_guard<std::mutex> lock(sv->mu);
-            if (!sv->pending.empty()) next = sv->pending.begin()->first;
+            int32_t n = 0;
+            if (int st = tick_locked(sv, &n)) return st;
+            if (n > 0) continue;
+            if (!sv->inflight.empty()) {
+                if (int st = drain(sv, 1)) return st;  // wait for the oldest round
+                continue;
+            }
+            if (all_done(sv)) return FS_OK;
+            if (sv->pending.empty()) {
+                if (!sv->waiting.empty()) return sv->fail(FS_ERR_OUT_OF_MEMORY, "a waiting request cannot be admitted");
+                return FS_OK;
+            }
+            wait = sv->pending.begin()->first - sv->now();
         }
-        if (next < 0.0) {
-            if (!sv->waiting.empty()) return sv->fail(FS_ERR_OUT_OF_MEMORY, "a waiting request cannot be admitted");
-            return FS_OK;
-        }
-        const double wait = next - sv->now();
+        // idle until the next arrival, without the lock so submissions get in
         if (wait > 0.0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
     }
 }
 
 int fs_server_status(fs_server* sv, uint64_t id, fs_request_status* out) {
     if (!sv) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null server");
+    std::lock_guard<std::mutex> lock(sv->mu);
     if (!out) return sv->fail(FS_ERR_INVALID_ARGUMENT, "null output");
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) return sv->fail(FS_ERR_INVALID_ARGUMENT, "unknown request id");
@@ -465,6 +480,8 @@ int fs_server_status(fs_server* sv, uint64_t id, fs_request_status* out) {
     s.state = q.state;
     s.steps_done = q.completed;
     s.frames = q.frames;
+    s.frame_w = q.r.desc.frame_w;
+    s.frame_h = q.r.desc.frame_h;
     s.arrival_s = q.arrival;
     s.admitted_s = q.admitted;
     s.finished_s = q.finished;
@@ -478,6 +495,7 @@ int fs_server_status(fs_server* sv, uint64_t id, fs_request_status* out) {
 
 int fs_server_frame(fs_server* sv, uint64_t id, uint32_t* bgra, int32_t* frameStep) {
     if (!sv) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null server");
+    std::lock_guard<std::mutex> lock(sv->mu);
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) return sv->fail(FS_ERR_INVALID_ARGUMENT, "unknown request id");
     const Request& q = it->second;
@@ -490,6 +508,7 @@ int fs_server_frame(fs_server* sv, uint64_t id, uint32_t* bgra, int32_t* frameSt
 
 int fs_server_take_scene(fs_server* sv, uint64_t id, fs_scene** out) {
     if (!sv) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null server");
+    std::lock_guard<std::mutex> lock(sv->mu);
     if (!out) return sv->fail(FS_ERR_INVALID_ARGUMENT, "null output");
     auto it = sv->reqs.find(id);
     if (it == sv->reqs.end()) return sv->fail(FS_ERR_INVALID_ARGUMENT, "unknown request id");
@@ -503,6 +522,7 @@ int fs_server_take_scene(fs_server* sv, uint64_t id, fs_scene** out) {
 
 int fs_server_stats_get(fs_server* sv, fs_server_stats* out) {
     if (!sv) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null server");
+    std::lock_guard<std::mutex> lock(sv->mu);
     if (!out) return sv->fail(FS_ERR_INVALID_ARGUMENT, "null output");
     fs_server_stats s{};
     s.requests_submitted = sv->submitted;
@@ -535,13 +555,18 @@ int fs_server_stats_get(fs_server* sv, fs_server_stats* out) {
 
 int fs_server_step_latencies(fs_server* sv, double* outMs, int64_t capacity, int64_t* count) {
     if (!sv) return fsi::global_fail(FS_ERR_NOT_INITIALIZED, "null server");
+    std::lock_guard<std::mutex> lock(sv->mu);
     const int64_t n = static_cast<int64_t>(sv->latencies.size());
     if (count) *count = n;
     if (outMs) std::copy(sv->latencies.begin(), sv->latencies.begin() + std::min(n, std::max<int64_t>(capacity, 0)), outMs);
     return FS_OK;
 }
 
-double fs_server_clock(fs_server* sv) { return sv ? sv->now() : 0.0; }
+double fs_server_clock(fs_server* sv) {
+    if (!sv) return 0.0;
+    std::lock_guard<std::mutex> lock(sv->mu);
+    return sv->now();
+}
 
 const char* fs_server_last_error(const fs_server* sv) { return sv ? sv->lastError.c_str() : "null server"; }
 

@@ -111,12 +111,12 @@ class Request(Structure):
 
 
 class RequestStatus(Structure):
-    _fields_ = [("state", c_int32), ("steps_done", c_int32), ("frames", c_int32), ("reserved", c_int32),
+    _fields_ = [("state", c_int32), ("steps_done", c_int32), ("frames", c_int32), ("frame_w", c_int32), ("frame_h", c_int32),
                 ("arrival_s", c_double), ("admitted_s", c_double), ("finished_s", c_double),
                 ("step_ms_p50", c_double), ("step_ms_p99", c_double), ("step_ms_max", c_double)]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class ServerStats(Structure):
@@ -656,8 +656,10 @@ class Server:
         self._check(self.lib.fs_server_status(self.h, rid, byref(s)), "fs_server_status")
         return s.as_dict()
 
-    def frame(self, rid: int, w: int, h: int):
-        out = np.empty((h, w), dtype=np.uint32)
+    def frame(self, rid: int):
+        """Latest frame of a request, sized from the request's own descriptor."""
+        st = self.status(rid)
+        out = np.empty((st["frame_h"], st["frame_w"]), dtype=np.uint32)
         step = c_int32()
         self._check(self.lib.fs_server_frame(self.h, rid, out.ctypes.data_as(POINTER(c_uint32)), byref(step)), "fs_server_frame")
         return out, step.value

@@ -148,3 +148,28 @@ def test_batch_argument_checks(hip_lib):
     b2.close()
     s32.close()
     s16.close()
+
+
+@pytest.mark.parametrize("first,second", [(firesim.STORAGE_FP32, firesim.STORAGE_BF16), (firesim.STORAGE_BF16, firesim.STORAGE_FP32)])
+def test_batch_storage_switch_after_empty(hip_lib, first, second):
+    """A batch that stepped scenes of one storage mode, emptied, and then takes a
+    scene of the other mode steps it with that mode's passes (the captured step
+    graphs are keyed by storage mode): bit for bit the scene stepped alone."""
+    grid = (24, 28, 20, 7, 3)
+    a = make(grid, 41, first)
+    b_alone, b_batched = make(grid, 42, second), make(grid, 42, second)
+    with firesim.Batch() as batch:
+        batch.set_tuning(sor_kernel="tb_t24_z10_k4", use_graph=True)
+        batch.add(a)
+        batch.step(2)
+        batch.remove(a)
+        batch.add(b_batched)
+        batch.step(3)
+        batch.synchronize()
+        b_alone.step(3)
+        assert_same(b_alone, b_batched, "second storage mode after the switch")
+        a_alone = make(grid, 41, first)
+        a_alone.step(2)
+        assert_same(a_alone, a, "first storage mode before the switch")
+    for s in (a, a_alone, b_alone, b_batched):
+        s.close()

@@ -51,7 +51,7 @@ def test_request_alone_equals_in_mix(hip_lib, storage):
         sv.run()
         st = sv.status(rid)
         assert st["state"] == firesim.REQ_DONE and st["steps_done"] == 25 and st["frames"] == 5
-        frame_alone, fstep = sv.frame(rid, 48, 32)
+        frame_alone, fstep = sv.frame(rid)
         assert fstep == 25
         a = sv.take_scene(rid)
     mix = [((24, 24, 24), 1, 40, 13), ((64, 32, 48), 2, 12, 30), ((32, 64, 32), 3, 30, 7), ((16, 16, 16), 4, 60, 1),
@@ -63,7 +63,7 @@ def test_request_alone_equals_in_mix(hip_lib, storage):
         stats = sv.stats()
         assert stats["requests_done"] == len(mix) + 1 and stats["requests_failed"] == 0
         assert stats["max_active_seen"] >= 2  # it really ran in a mix
-        frame_mix, _ = sv.frame(rid, 48, 32)
+        frame_mix, _ = sv.frame(rid)
         m = sv.take_scene(rid)
         for i in ids:
             assert sv.status(i)["steps_done"] == [x[2] for x in mix][ids.index(i)]
