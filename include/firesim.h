@@ -243,6 +243,27 @@ const char* fs_last_global_error(void);
  * achievable ceiling reported beside the nominal HBM peak. */
 int fs_copy_bandwidth(int32_t device, uint64_t bytes, int32_t reps, double* gbps);
 
+/* ---- pressure pass without allocation ----
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, serving loops and harnesses that
+ * forbid hipMalloc. The pressure pass of fs_run_stage(FS_STAGE_PRESSURE)
+ * (red-black SOR, :1201-1227, the same per-cell FP32 update and red/black
+ * order, so bit-identical results) on caller device buffers: p (nx*ny*nz
+ * floats, x fastest) holds the initial guess and receives the solution, d
+ * (nx*ny*nz floats) is the right-hand side. Enqueued on `stream` (a
+ * hipStream_t; null is the null stream) of `device`, which becomes the
+ * calling thread's current device; returns without waiting. The tile kernels
+ * ping-pong with a library-owned static device array of FS_NOALLOC_MAX_CELLS
+ * floats (64 MiB, held by every GPU the library runs on until the process
+ * ends) instead of a second buffer; grids of more cells are rejected with
+ * FS_ERR_INVALID_ARGUMENT. The call makes no hipMalloc, hipFree, memset or
+ * other allocating call. Every call on one GPU shares that array: calls that
+ * overlap on one GPU (on different streams, or in different threads without
+ * ordering) race and give wrong results. */
+#define FS_NOALLOC_MAX_CELLS (1u << 24)
+int fs_pressure_solve_noalloc(float* p, const float* d, int32_t nx, int32_t ny, int32_t nz, float omega, int32_t iterations,
+                              int32_t sor_kernel, int32_t device, void* stream);
+
 /* ---- scene lifecycle ---- */
 int fs_create(const fs_scene_desc* desc, fs_scene** out);
 /* fs_create in caller-owned device memory (256-byte aligned, at least

@@ -153,6 +153,13 @@ hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_
 hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalize = true);
 hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalizeRelease = false);
 hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st);
+// The pressure pass on bare buffers: p holds the initial guess, p2 is the
+// second buffer the tile kernels ping-pong with (unused by the half-sweep
+// kernel). *resultInP2 tells where the solution ends (an odd launch count).
+hipError_t launch_pressure_buffers(float* p, float* p2, const float* d, int nx, int ny, int nz, float omega, int iterations, int sorKernel,
+                                   hipStream_t st, bool* resultInP2);
+// n floats from a to b (n < 2^30), a plain kernel (no memset or copy node).
+hipError_t launch_copy_f32(const float* a, float* b, size_t n, hipStream_t st);
 hipError_t launch_project(DeviceScene& s, const DevParams& p, hipStream_t st);
 // bytes / 16 float4 elements from a to b (bytes a multiple of 16).
 hipError_t launch_copy4(const void* a, void* b, size_t bytes, hipStream_t st);

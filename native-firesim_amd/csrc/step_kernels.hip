@@ -1009,41 +1009,66 @@ int sor_depth(int sorKernel) { return sor_shape(sorKernel).k; }
 int sor_tile_edge(int sorKernel) { return sor_shape(sorKernel).t; }
 int sor_tile_depth(int sorKernel) { return sor_shape(sorKernel).tz; }
 
+// Tiles read the halo of their neighbours, so a launch must not overwrite its
+// own input: launch j reads buffer a (j even) or b (j odd) and writes the other.
+// Returns the number of launches.
 template <int T, int K, int TZ = T>
-static void launch_sor_tb(DeviceScene& s, const DevParams& p, hipStream_t st) {
-    const dim3 g((s.nx + T - 1) / T, (s.ny + T - 1) / T, (s.nz + TZ - 1) / TZ);
-    const int half = 2 * p.iterations;
+static int launch_sor_tb(float* a, float* b, const float* d, int nx, int ny, int nz, float omega, int iterations, hipStream_t st) {
+    const dim3 g((nx + T - 1) / T, (ny + T - 1) / T, (nz + TZ - 1) / TZ);
+    const int half = 2 * iterations;
     constexpr int E = T + 2 * K;
-    // Tiles read the halo of their neighbours, so a launch must not overwrite
-    // its own input: each launch reads one pressure buffer and writes the other.
-    for (int h0 = 0; h0 < half; h0 += K) {
-        k_sor_tb<T, K, TZ><<<g, E * E, 0, st>>>(s.p, s.p2, s.d, p.omega, h0, min(K, half - h0), s.nx, s.ny, s.nz);
-        std::swap(s.p, s.p2);
+    int launches = 0;
+    for (int h0 = 0; h0 < half; h0 += K, ++launches) {
+        k_sor_tb<T, K, TZ><<<g, E * E, 0, st>>>(a, b, d, omega, h0, min(K, half - h0), nx, ny, nz);
+        std::swap(a, b);
     }
+    return launches;
 }
 
-hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
-    switch (s.sorKernel) {
-        case 1: launch_sor_tb<16, 4>(s, p, st); break;
-        case 2: launch_sor_tb<24, 4>(s, p, st); break;
-        case 3: launch_sor_tb<24, 4, 8>(s, p, st); break;
-        case 4: launch_sor_tb<24, 4, 16>(s, p, st); break;
-        case 5: launch_sor_tb<24, 4, 28>(s, p, st); break;
-        case 6: launch_sor_tb<24, 4, 10>(s, p, st); break;
-        case 7: launch_sor_tb<24, 4, 12>(s, p, st); break;
-        case 8: launch_sor_tb<24, 4, 6>(s, p, st); break;
-        case 9: launch_sor_tb<26, 3, 13>(s, p, st); break;
-        case 10: launch_sor_tb<26, 3, 9>(s, p, st); break;
-        case 11: launch_sor_tb<22, 5, 8>(s, p, st); break;
+hipError_t launch_pressure_buffers(float* p, float* p2, const float* d, int nx, int ny, int nz, float omega, int iterations, int sorKernel,
+                                   hipStream_t st, bool* resultInP2) {
+    int launches = 0;
+    switch (sorKernel) {
+        case 1: launches = launch_sor_tb<16, 4>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 2: launches = launch_sor_tb<24, 4>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 3: launches = launch_sor_tb<24, 4, 8>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 4: launches = launch_sor_tb<24, 4, 16>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 5: launches = launch_sor_tb<24, 4, 28>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 6: launches = launch_sor_tb<24, 4, 10>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 7: launches = launch_sor_tb<24, 4, 12>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 8: launches = launch_sor_tb<24, 4, 6>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 9: launches = launch_sor_tb<26, 3, 13>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 10: launches = launch_sor_tb<26, 3, 9>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 11: launches = launch_sor_tb<22, 5, 8>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
         default: {
             const dim3 b = cell_block();
-            const dim3 g = sor_grid(0, s.nx, s.ny, s.nz);
-            for (int i = 0; i < p.iterations; ++i) {
-                k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 0, p.omega, s.nx, s.ny, s.nz);
-                k_sor_half<<<g, b, 0, st>>>(s.p, s.d, 1, p.omega, s.nx, s.ny, s.nz);
+            const dim3 g = sor_grid(0, nx, ny, nz);
+            for (int i = 0; i < iterations; ++i) {
+                k_sor_half<<<g, b, 0, st>>>(p, d, 0, omega, nx, ny, nz);
+                k_sor_half<<<g, b, 0, st>>>(p, d, 1, omega, nx, ny, nz);
             }
         }
     }
+    *resultInP2 = (launches & 1) != 0;
+    return hipGetLastError();
+}
+
+hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
+    bool inP2 = false;
+    const hipError_t e = launch_pressure_buffers(s.p, s.p2, s.d, s.nx, s.ny, s.nz, p.omega, p.iterations, s.sorKernel, st, &inP2);
+    if (inP2) std::swap(s.p, s.p2);
+    return e;
+}
+
+__global__ __launch_bounds__(256) void k_copy_f32(const float* __restrict__ a, float* __restrict__ b, Idx n) {
+    const Idx i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) st_at(b, i, ld_at(a, i));
+}
+
+hipError_t launch_copy_f32(const float* a, float* b, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (n >= (static_cast<size_t>(1) << 30)) return hipErrorInvalidValue;
+    k_copy_f32<<<static_cast<unsigned>((n + 255) / 256), 256, 0, st>>>(a, b, static_cast<Idx>(n));
     return hipGetLastError();
 }
 

@@ -308,6 +308,8 @@ def load_library(path: str = LIB_PATH):
         "fs_server_step_latencies": (c_int32, [c_void_p, P(c_double), c_int64, P(c_int64)]),
         "fs_server_clock": (c_double, [c_void_p]),
         "fs_server_last_error": (c_char_p, [c_void_p]),
+        "fs_pressure_solve_noalloc": (c_int32, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_float, c_int32, c_int32, c_int32,
+                                                c_void_p]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
