@@ -28,8 +28,10 @@
 //  * wind_z drives w with the same height profile the reference uses for
 //    wind on u (:1179); SOR omega and iteration count are parameters.
 //
-// Parity status of each function is listed in the header of tests/test_oracle.py
-// (which pins them) and in the repository design notes.
+// Every function is pinned by tests/test_oracle.py (primitives) and
+// tests/test_oracle_pins.py (passes: closed forms and independent numpy
+// restatements from src/fire_cuda.cu); oracle/mutation_sweep.py checks that
+// one-constant slips fail those tests.
 
 #include "../include/firesim.h"
 
@@ -989,9 +991,8 @@ static inline R domain_fade(R u, R v, R w) {
 
 // buildSceneLightKernel :1445-1525 then propagateSceneLightKernel ×2 :1527-1565
 // with (0.48, 0.92) then (0.34, 0.96) (:3313-3314).
-static void light_pass(State& st, const P& p) {
-    const size_t lc = st.lcells();
-    std::vector<R> L(4 * lc);
+// buildSceneLightKernel :1445-1525: radiance + extinction into L, shadow into st.shadow.
+static void light_build(State& st, const P& p, std::vector<R>& L) {
     const R* F[10];
     for (int f = 0; f < 10; ++f) F[f] = st.s[f].data();
 #pragma omp parallel for schedule(static)
@@ -1048,7 +1049,11 @@ static void light_pass(State& st, const P& p) {
             }
         }
     }
-    auto propagate = [&](const std::vector<R>& in, std::vector<R>& out, R diffusion, R decay) {
+}
+
+// propagateSceneLightKernel :1527-1565, in -> out.
+static void light_propagate(const State& st, const std::vector<R>& in, std::vector<R>& out, R diffusion, R decay) {
+    {
 #pragma omp parallel for schedule(static)
         for (int lz = 0; lz < st.lnz; ++lz) {
             for (int ly = 0; ly < st.lny; ++ly) {
@@ -1082,10 +1087,16 @@ static void light_pass(State& st, const P& p) {
                 }
             }
         }
-    };
-    std::vector<R> tmp(4 * lc);
-    propagate(L, tmp, 0.48f, 0.92f);
-    propagate(tmp, st.light, 0.34f, 0.96f);
+    }
+}
+
+// Build, then two propagations (0.48, 0.92) and (0.34, 0.96) (:3313-3314).
+static void light_pass(State& st, const P& p) {
+    const size_t lc = st.lcells();
+    std::vector<R> L(4 * lc), tmp(4 * lc);
+    light_build(st, p, L);
+    light_propagate(st, L, tmp, 0.48f, 0.92f);
+    light_propagate(st, tmp, st.light, 0.34f, 0.96f);
 }
 
 // -------------------------------------------------------------- render ----
@@ -1983,5 +1994,54 @@ void or_camera_response(const double* rgb, double exposure, double* out) {
     out[0] = c.x; out[1] = c.y; out[2] = c.z;
 }
 double or_domain_fade(double u, double v, double w) { return domain_fade(u, v, w); }
+
+// MacCormack sample of a (nz, ny, nx) scalar array f at backtrace (pu, pv, pw)
+// with the previous-position velocity pvel (3) and timestep dt.
+double or_maccormack(const double* f, int nx, int ny, int nz, double dt, double current, const double* pvel, double pu, double pv, double pw,
+                     double blend) {
+    State st;
+    st.nx = nx; st.ny = ny; st.nz = nz;
+    P p;
+    p.dt = dt;
+    return maccormack(st, p, f, current, v3(pvel[0], pvel[1], pvel[2]), pu, pv, pw, blend);
+}
+double or_capsule_mask(double x, double z, double cx, double cz, double angle, double halfLength, double radius) {
+    return capsule_mask(x, z, cx, cz, angle, halfLength, radius);
+}
+// Bed material and source of the scene's current parameters at world (x, z).
+double or_bed_material(void* h, double x, double z) {
+    const State& st = *static_cast<State*>(h);
+    return bed_material(x, z, make_p(st, st.time));
+}
+double or_bed_source(void* h, double x, double z, double height, double time) {
+    const State& st = *static_cast<State*>(h);
+    return bed_source(x, z, height, make_p(st, time));
+}
+// The light build alone (light <- radiance + extinction, shadow), at the scene time.
+void or_light_build(void* h) {
+    State& st = *static_cast<State*>(h);
+    std::vector<R> L(4 * st.lcells());
+    light_build(st, make_p(st, st.time), L);
+    st.light.swap(L);
+}
+// One light propagation of the current light volume, in place.
+void or_light_propagate(void* h, double diffusion, double decay) {
+    State& st = *static_cast<State*>(h);
+    std::vector<R> out(4 * st.lcells());
+    light_propagate(st, st.light, out, diffusion, decay);
+    st.light.swap(out);
+}
+// Room colour along ray rd from the scene camera, for pixel uv (render time clock).
+void or_room_background(void* h, double uvx, double uvy, const double* rd, double glow, double* rgb) {
+    const State& st = *static_cast<State*>(h);
+    const V3 c = room_background(st, make_p(st, st.renderTime), uvx, uvy, v3(rd[0], rd[1], rd[2]), glow);
+    rgb[0] = c.x; rgb[1] = c.y; rgb[2] = c.z;
+}
+// The scene camera's eye position (the origin of or_room_background rays).
+void or_camera_eye(void* h, double* eye) {
+    const State& st = *static_cast<State*>(h);
+    const Cam c = make_camera(st.prm, st.fw, st.fh);
+    eye[0] = c.eye.x; eye[1] = c.eye.y; eye[2] = c.eye.z;
+}
 
 }  // extern "C"

@@ -69,6 +69,14 @@ def load():
         "or_aces": (c_double, [c_double]),
         "or_camera_response": (None, [D, c_double, D]),
         "or_domain_fade": (c_double, [c_double, c_double, c_double]),
+        "or_maccormack": (c_double, [D, c_int, c_int, c_int, c_double, c_double, D, c_double, c_double, c_double, c_double]),
+        "or_capsule_mask": (c_double, [c_double] * 7),
+        "or_bed_material": (c_double, [c_void_p, c_double, c_double]),
+        "or_bed_source": (c_double, [c_void_p, c_double, c_double, c_double, c_double]),
+        "or_light_build": (None, [c_void_p]),
+        "or_light_propagate": (None, [c_void_p, c_double, c_double]),
+        "or_room_background": (None, [c_void_p, c_double, c_double, D, c_double, D]),
+        "or_camera_eye": (None, [c_void_p, D]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -29,6 +29,10 @@ function is pinned by a check that a plausible mistake would fail:
         debug modes; campfire/burner spawn gates on the right hash (k=2 / k=0);
         each spark is rank-1 in its colour, between the hot and cooled colours
 
+The passes beyond these primitives (advect-react term by term, force, light,
+metrics, beds, reset, room colour, raymarch, embers, tonemap) are pinned in
+test_oracle_pins.py.
+
 "Parity unpinned" items: none of the above is compared against the reference's
 own sin-hash noise (not portable, replaced by the integer hash by design).
 """
