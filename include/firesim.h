@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FS_ABI_VERSION 1
+#define FS_ABI_VERSION 2
 
 enum fs_status {
     FS_OK = 0,
@@ -67,7 +67,8 @@ enum fs_field_id {
     FS_FIELD_DIVERGENCE = 14,
     FS_FIELD_LIGHT = 15,
     FS_FIELD_SHADOW = 16,
-    FS_FIELD_COUNT = 17
+    FS_FIELD_DIR_TRANSMITTANCE = 17, /* light grid; written by the light stage while dir_light is on */
+    FS_FIELD_COUNT = 18
 };
 
 /* Passes of one step, in the order fs_step runs them (src/fire_cuda.cu:3196-3318). */
@@ -144,6 +145,20 @@ typedef struct fs_params {
     int32_t sor_iterations;      /* [1, 1000]; reference constant 40 (:104) */
     uint32_t noise_seed;         /* seed of the integer-hash noise */
     int32_t ember_count;         /* [0, 192] sparks splatted into the HDR frame (:2323-2452) */
+    /* Extensions without a reference counterpart unless noted; 0 keeps the
+     * reference behaviour. */
+    int32_t closed_top;          /* 0/1: 1 makes the top v face a wall like the floor (the reference's top is open, :905-908, :1275-1281) */
+    int32_t noise_forcing;       /* 0: the reference's fbm noise forcing (:1164-1165); 1: divergence-free curl noise of
+                                  * amplitude curl_noise in its place */
+    float curl_noise;            /* [0, 4]: with noise_forcing 1, faces accelerate by curl_noise * (curl of the potential)
+                                  * per second; 0 gives no noise forcing at all */
+    int32_t noise_mode;          /* 0: integer-hash noise (default, portable); 1: the reference's sin hash (:381-443),
+                                  * a visual mode whose values depend on the platform's sinf */
+    int32_t show_gizmos;         /* 0/1: draw the source ring and axis gizmos on the BGRA frame (drawGizmos :1931-1951) */
+    int32_t active_gizmo;        /* any value; 1 highlights the source ring (FireSettings::activeGizmo) */
+    int32_t dir_light;           /* 0/1: a directional key light scattered by the smoke and lighting the room */
+    float dir_light_x, dir_light_y, dir_light_z; /* direction the light travels (normalized; zero gives straight down) */
+    float dir_light_r, dir_light_g, dir_light_b; /* [0, 20] radiance */
 } fs_params;
 
 /* Spherical fuel + heat source in voxel units (cell i has centre i + 0.5).

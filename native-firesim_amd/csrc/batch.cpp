@@ -100,7 +100,8 @@ int build_table(const fs_batch* b, int sorKernel, BatchTable* t) {
         const int launches = launches_of(e.halfSweeps, sorKernel);
         e.pFinal = (tiled && (launches & 1)) ? d.p2 : d.p;
         e.d = d.d; e.cu = d.cu; e.cv = d.cv; e.cw = d.cw;
-        e.light = d.light; e.lightNext = d.lightNext; e.shadow = d.shadow;
+        for (int k = 0; k < 3; ++k) e.psi[k] = d.psi[k];
+        e.light = d.light; e.lightNext = d.lightNext; e.shadow = d.shadow; e.dirT = d.dirT;
         e.partials = d.partials; e.acc = d.acc; e.clock = d.clock;
         const dim3 cg = fs::cell_grid(d.nx, d.ny, d.nz), fg = fs::cell_grid(d.nx, d.ny + 1, d.nz + 1);  // face passes: x = nx - 1 also covers x = nx
         const dim3 sg = fs::sor_grid(sorKernel, d.nx, d.ny, d.nz), lg = fs::light_grid(d.lnx, d.lny, d.lnz);

@@ -63,6 +63,8 @@ fs_params to_params(const FireSettings& s) {
     p.raymarch_steps = clampi(s.raymarchSteps, 24, 120);
     p.ember_count = clampi(s.emberCount, 0, 192);
     p.render_debug_mode = clampi(s.renderDebugMode, 0, 6);
+    p.show_gizmos = s.showGizmos != 0 ? 1 : 0;
+    p.active_gizmo = s.activeGizmo;
     p.exposure = clampf(s.exposure, 0.45f, 2.40f);
     p.reflection_gain = clampf(s.reflectionGain, 0.0f, 2.2f);
     p.smoke_darkness = clampf(s.smokeDarkness, 0.35f, 2.4f);

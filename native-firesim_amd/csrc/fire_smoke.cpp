@@ -40,7 +40,7 @@ int main(int argc, char** argv) {
         s.cameraYaw = 0.18f;
         s.cameraPitch = 0.08f;
         s.cameraDistance = 2.62f;
-        s.showGizmos = 0;
+        s.showGizmos = 0;  // the frame call gets simSettings.showGizmos = 0 (src/main.cpp:4204-4206)
         ok = (i == 63) ? fireCudaStepAndRenderMeasured(frame.data(), s, &metrics) : fireCudaStepAndRender(frame.data(), s);
         if (!ok) std::fprintf(stderr, "frame %d failed: %s\n", i, fireCudaLastError());
     }

@@ -16,6 +16,13 @@ namespace fs {
 enum ScalarId { HEAT = 0, FUEL, OXYGEN, SOOT, CHARF, ASH, PYRO, PROGRESS, TURB, SOPT, NSCALAR };
 
 constexpr int kMaxEmitters = 16;
+
+// Seed of one noise stream, and which lattice hash it uses: the PCG integer
+// hash (default) or the reference's sin hash (a visual mode).
+struct NoiseKey {
+    uint32_t salt;
+    int sinHash;
+};
 constexpr int kMaxEmbers = 192;  // kMaxEmberCount, src/fire_cuda.cu:110
 constexpr float kAdvectionScale = 0.62f;  // src/fire_cuda.cu:107
 
@@ -35,8 +42,16 @@ struct DevParams {
     float burnerX[4], burnerY[4], burnerZ[4];
     float omega;
     int iterations;
-    uint32_t saltNoise2, saltNoise3, saltJitter, saltDither, saltGrain, saltEmber;
+    NoiseKey saltNoise2, saltNoise3, saltJitter, saltDither, saltGrain, saltEmber;
+    NoiseKey saltCurl[3];  // the three components of the curl-noise potential
     int emberCount;
+    int closedTop;        // 1: the top v face is a wall
+    int curlOn;           // 1: divergence-free curl noise replaces the fbm noise terms
+    float curlNoise;      // its amplitude
+    int showGizmos, activeGizmo;
+    int dirLight;
+    float dirLightDir[3];    // unit direction the light travels
+    float dirLightColor[3];
     float camEye[3], camForward[3], camRight[3], camUp[3];
     float tanHalfFov, aspect;
     int emitterCount;
@@ -144,11 +159,25 @@ __device__ __forceinline__ uint32_t pcg(uint32_t v) {
     return (word >> 22u) ^ word;
 }
 __device__ __forceinline__ float unit24(uint32_t h) { return static_cast<float>(h >> 8) * (1.0f / 16777216.0f); }
-__device__ __forceinline__ float hash2(int x, int y, uint32_t salt) {
-    return unit24(pcg(pcg(salt + static_cast<uint32_t>(y)) + static_cast<uint32_t>(x)));
+
+// The reference's lattice hashes hash21/hash31 (src/fire_cuda.cu:381-388):
+// fract(sin(.) * 43758.5453) in FP32. Their values depend on the platform's
+// sinf, so they serve only the visual noise mode (NoiseKey::sinHash).
+__device__ __forceinline__ float fract(float v) { return v - floorf(v); }
+__device__ __forceinline__ float sin_hash21(float x, float y) {
+    const float a = x * 127.1f + y * 311.7f, b = x * 269.5f + y * 183.3f;
+    return fract(sinf(a + b) * 43758.5453123f);
 }
-__device__ __forceinline__ float hash3(int x, int y, int z, uint32_t salt) {
-    return unit24(pcg(pcg(pcg(salt + static_cast<uint32_t>(z)) + static_cast<uint32_t>(y)) + static_cast<uint32_t>(x)));
+__device__ __forceinline__ float sin_hash31(float x, float y, float z) { return fract(sinf(x * 127.1f + y * 311.7f + z * 74.7f) * 43758.5453123f); }
+
+__device__ __forceinline__ float hash2(int x, int y, NoiseKey k) {
+    if (k.sinHash) return sin_hash21(static_cast<float>(x), static_cast<float>(y));
+    return unit24(pcg(pcg(k.salt + static_cast<uint32_t>(y)) + static_cast<uint32_t>(x)));
+}
+// A hash of integer inputs (xi, yi) that the reference takes at the float
+// inputs (xf, yf) (pixel jitter, dither, grain, spark seeds).
+__device__ __forceinline__ float hash2f(float xf, float yf, int xi, int yi, NoiseKey k) {
+    return k.sinHash ? sin_hash21(xf, yf) : hash2(xi, yi, k);
 }
 
 // Value noise, reference interpolation src/fire_cuda.cu:390-399: the four
@@ -159,12 +188,19 @@ __device__ __forceinline__ float vblend2(float a, float b, float c, float d, flo
     const float uy = fy * fy * (3.0f - 2.0f * fy);
     return lerpf(lerpf(a, b, ux), lerpf(c, d, ux), uy);
 }
-__device__ inline float vnoise2(float px, float py, uint32_t salt) {
+// The reference's sin-hash value noise (:390-399), for the visual mode.
+__device__ __forceinline__ float vnoise2_sin(float px, float py) {
+    const float fx0 = floorf(px), fy0 = floorf(py);
+    return vblend2(sin_hash21(fx0, fy0), sin_hash21(fx0 + 1.0f, fy0), sin_hash21(fx0, fy0 + 1.0f), sin_hash21(fx0 + 1.0f, fy0 + 1.0f), px - fx0,
+                   py - fy0);
+}
+__device__ inline float vnoise2(float px, float py, NoiseKey k) {
+    if (k.sinHash) return vnoise2_sin(px, py);
     const float fx0 = floorf(px), fy0 = floorf(py);
     const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
     // one row hash per lattice y, shared by both x corners
-    const uint32_t r0 = pcg(salt + static_cast<uint32_t>(iy));
-    const uint32_t r1 = pcg(salt + static_cast<uint32_t>(iy + 1));
+    const uint32_t r0 = pcg(k.salt + static_cast<uint32_t>(iy));
+    const uint32_t r1 = pcg(k.salt + static_cast<uint32_t>(iy + 1));
     return vblend2(lattice2(r0, ix), lattice2(r0, ix + 1), lattice2(r1, ix), lattice2(r1, ix + 1), px - fx0, py - fy0);
 }
 
@@ -181,11 +217,13 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
 }
 // Lanes on one lattice row (same floor(py)): the two row hashes are
 // wave-uniform (scalar unit), lane L hashes columns base + L of both rows.
-__device__ __forceinline__ float vnoise2_row_wave(float px, float py, uint32_t salt) {
+__device__ __forceinline__ float vnoise2_row_wave(float px, float py, NoiseKey k) {
+    if (k.sinHash) return vnoise2_sin(px, py);
+    const uint32_t salt = k.salt;
     const float fx0 = floorf(px), fy0 = floorf(py);
     const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
     const int iyw = __builtin_amdgcn_readfirstlane(iy);
-    if (!full_wave() || !__all(iy == iyw)) return vnoise2(px, py, salt);
+    if (!full_wave() || !__all(iy == iyw)) return vnoise2(px, py, k);
     const uint32_t r0 = pcg(salt + static_cast<uint32_t>(iyw));
     const uint32_t r1 = pcg(salt + static_cast<uint32_t>(iyw + 1));
     const int base = __builtin_amdgcn_readfirstlane(ix);
@@ -198,7 +236,9 @@ __device__ __forceinline__ float vnoise2_row_wave(float px, float py, uint32_t s
 }
 // Lanes on one lattice column (same floor(px)): lane L hashes row base + L
 // and its two column values.
-__device__ __forceinline__ float vnoise2_col_wave(float px, float py, uint32_t salt) {
+__device__ __forceinline__ float vnoise2_col_wave(float px, float py, NoiseKey k) {
+    if (k.sinHash) return vnoise2_sin(px, py);
+    const uint32_t salt = k.salt;
     const float fx0 = floorf(px), fy0 = floorf(py);
     const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0);
     const int ixw = __builtin_amdgcn_readfirstlane(ix);
@@ -209,17 +249,25 @@ __device__ __forceinline__ float vnoise2_col_wave(float px, float py, uint32_t s
         const float g0 = lattice2(r, ixw), g1 = lattice2(r, ixw + 1);
         return vblend2(lane_value(g0, off), lane_value(g1, off), lane_value(g0, off + 1), lane_value(g1, off + 1), px - fx0, py - fy0);
     }
-    return vnoise2(px, py, salt);
+    return vnoise2(px, py, k);
 }
 
 // src/fire_cuda.cu:401-421
-__device__ inline float vnoise3(float px, float py, float pz, uint32_t salt) {
+__device__ inline float vnoise3(float px, float py, float pz, NoiseKey k) {
     const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
     const int ix = static_cast<int>(fx0), iy = static_cast<int>(fy0), iz = static_cast<int>(fz0);
     const float fx = px - fx0, fy = py - fy0, fz = pz - fz0;
     const float ux = fx * fx * (3.0f - 2.0f * fx);
     const float uy = fy * fy * (3.0f - 2.0f * fy);
     const float uz = fz * fz * (3.0f - 2.0f * fz);
+    if (k.sinHash) {
+        const float c00 = lerpf(sin_hash31(fx0, fy0, fz0), sin_hash31(fx0 + 1.0f, fy0, fz0), ux);
+        const float c10 = lerpf(sin_hash31(fx0, fy0 + 1.0f, fz0), sin_hash31(fx0 + 1.0f, fy0 + 1.0f, fz0), ux);
+        const float c01 = lerpf(sin_hash31(fx0, fy0, fz0 + 1.0f), sin_hash31(fx0 + 1.0f, fy0, fz0 + 1.0f), ux);
+        const float c11 = lerpf(sin_hash31(fx0, fy0 + 1.0f, fz0 + 1.0f), sin_hash31(fx0 + 1.0f, fy0 + 1.0f, fz0 + 1.0f), ux);
+        return lerpf(lerpf(c00, c10, uy), lerpf(c01, c11, uy), uz);
+    }
+    const uint32_t salt = k.salt;
     const uint32_t z0 = pcg(salt + static_cast<uint32_t>(iz));
     const uint32_t z1 = pcg(salt + static_cast<uint32_t>(iz + 1));
     const uint32_t r00 = pcg(z0 + static_cast<uint32_t>(iy));
@@ -242,7 +290,7 @@ __device__ inline float vnoise3(float px, float py, float pz, uint32_t salt) {
 // 0 per lane (vnoise2), 1 lanes on one lattice row, 2 lanes on one lattice
 // column (the wavefront forms above; same values).
 template <int WAVE>
-__device__ __forceinline__ float fbm2_t(float px, float py, uint32_t salt) {
+__device__ __forceinline__ float fbm2_t(float px, float py, NoiseKey salt) {
     float sum = 0.0f, amp = 0.5f;
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
@@ -257,10 +305,10 @@ __device__ __forceinline__ float fbm2_t(float px, float py, uint32_t salt) {
     }
     return sum;
 }
-__device__ inline float fbm2(float px, float py, uint32_t salt) { return fbm2_t<0>(px, py, salt); }
+__device__ inline float fbm2(float px, float py, NoiseKey salt) { return fbm2_t<0>(px, py, salt); }
 
 // 3-octave fbm, saturated, src/fire_cuda.cu:434-443
-__device__ inline float fbm3(float px, float py, float pz, uint32_t salt) {
+__device__ inline float fbm3(float px, float py, float pz, NoiseKey salt) {
     float sum = 0.0f, amp = 0.5f;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -417,7 +465,7 @@ __device__ inline float capsule_mask(float x, float z, float cx, float cz, float
 // Procedural bed material (src/fire_cuda.cu:746-793); scene 3 has none.
 __device__ inline float bed_material(float x, float z, const DevParams& p) {
     if (p.sceneId == 3) return 0.0f;
-    const uint32_t s2 = p.saltNoise2;
+    const NoiseKey s2 = p.saltNoise2;
     if (p.sceneId == 1) {
         const float logA = capsule_mask(x, z, -0.06f, 0.01f, 0.38f, 0.34f, 0.060f);
         const float logB = capsule_mask(x, z, 0.07f, -0.02f, -0.58f, 0.33f, 0.058f);

@@ -34,8 +34,10 @@ struct DeviceScene {
     float *p = nullptr, *d = nullptr;
     float* p2 = nullptr;  // second pressure buffer: tile SOR launches read one, write the other
     float *cu = nullptr, *cv = nullptr, *cw = nullptr;  // per-cell force contributions
+    float* psi[3] = {};  // curl-noise potential on grid edges, nx * (ny + 1) * nz each
     float4 *light = nullptr, *lightNext = nullptr;
     float* shadow = nullptr;
+    float* dirT = nullptr;  // transmittance toward the directional light, one per light cell
     float4* hdr = nullptr;
     uint32_t* bgra = nullptr;
     uint16_t* rgba16 = nullptr;    // FP16 RGBA frame, 4 halves per pixel
@@ -69,8 +71,10 @@ struct BatchScene {
     float *p0, *p1;       // pressure buffers in launch order (p0 zeroed by the force pass)
     float* pFinal;        // buffer holding the solution after the pressure pass
     float *d, *cu, *cv, *cw;
+    float* psi[3];
     float4 *light, *lightNext;
     float* shadow;
+    float* dirT;
     double* partials;
     double* acc;
     float* clock;

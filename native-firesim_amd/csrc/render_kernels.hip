@@ -101,6 +101,57 @@ __global__ __launch_bounds__(256) void k_light_propagate(float4* __restrict__ ou
                        blockIdx.z * blockDim.z + threadIdx.z);
 }
 
+// ----------------------------------------------------- directional light ----
+// An extension (no reference counterpart): a key light travelling along the
+// unit direction dirLightDir. Its transmittance to each light cell is
+// Beer-Lambert along the ray toward the light, exp(-kDirKappa * sum_k ds * w_k
+// * ext(s_k)), with samples s_k = (k + 1/2) ds (ds is half the smallest light
+// cell) and weights w_k = sat(1/2 + (len - s_k) / ds), len being the ray's
+// length inside the volume box. sum_k ds * w_k is exactly len (the weights are
+// the box tiling of [0, len)), so a uniform extinction e gives exp(-kDirKappa e len),
+// and the weights make the sum continuous in len. The smoke scatters the light
+// into the camera and the room surfaces receive it diffusely.
+constexpr float kDirKappa = 3.0f;    // extinction per world unit at light-volume extinction 1
+constexpr float kDirScatter = 2.0f;  // in-scatter gain of the smoke's single-scattering albedo per step
+constexpr float kDirDiffuse = 0.10f; // diffuse gain on the room surfaces
+constexpr int kDirMaxSamples = 208;  // box diagonal 3.35 / smallest ds 0.0171 = 196.1, + 1
+__device__ __forceinline__ float dir_step(const DevParams& p) {
+    return 0.5f * fminf(fminf(2.10f / static_cast<float>(p.lnx), 2.03f / static_cast<float>(p.lny)), 1.64f / static_cast<float>(p.lnz));
+}
+// Distance from x0 along d to the slab [lo, hi] (x0 inside); huge when d ~ 0.
+__device__ __forceinline__ float slab_exit(float x0, float d, float lo, float hi) {
+    if (d > 1.0e-6f) return (hi - x0) / d;
+    if (d < -1.0e-6f) return (lo - x0) / d;
+    return 1.0e30f;
+}
+__device__ __forceinline__ void dir_trans_at(float* __restrict__ DT, const float4* __restrict__ L, const DevParams& p, int lx, int ly, int lz) {
+    if (lx >= p.lnx || ly >= p.lny || lz >= p.lnz) return;
+    const float u = (static_cast<float>(lx) + 0.5f) / static_cast<float>(p.lnx);
+    const float v = (static_cast<float>(ly) + 0.5f) / static_cast<float>(p.lny);
+    const float w = (static_cast<float>(lz) + 0.5f) / static_cast<float>(p.lnz);
+    const float x0 = u * 2.10f - 1.05f, y0 = v * 2.03f + 0.02f, z0 = w * 1.64f - 0.82f;
+    const float dx = -p.dirLightDir[0], dy = -p.dirLightDir[1], dz = -p.dirLightDir[2];
+    const float len = fminf(fminf(slab_exit(x0, dx, -1.05f, 1.05f), slab_exit(y0, dy, 0.02f, 2.05f)), slab_exit(z0, dz, -0.82f, 0.82f));
+    const float ds = dir_step(p);
+    float tau = 0.0f;
+    for (int k = 0; k < kDirMaxSamples; ++k) {
+        const float s = ds * (static_cast<float>(k) + 0.5f);
+        const float wk = fminf(1.0f, 0.5f + (len - s) / ds);
+        if (wk <= 0.0f) break;
+        const float qx = x0 + dx * s, qy = y0 + dy * s, qz = z0 + dz * s;
+        tau += wk * sample_light(L, p, (qx + 1.05f) / 2.10f, (qy - 0.02f) / 2.03f, (qz + 0.82f) / 1.64f).w;
+    }
+    DT[(static_cast<size_t>(lz) * p.lny + ly) * p.lnx + lx] = expf(-kDirKappa * ds * tau);
+}
+__global__ __launch_bounds__(256) void k_dir_trans(float* __restrict__ DT, const float4* __restrict__ L, DevParams p) {
+    dir_trans_at(DT, L, p, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z * blockDim.z + threadIdx.z);
+}
+// Trilinear sample of the transmittance volume (the light grid's mapping).
+__device__ __forceinline__ float sample_dir_trans(const float* __restrict__ DT, const DevParams& p, float u, float v, float w) {
+    return tri_plain(DT, tri_setup(p.lnx, p.lny, p.lnz, static_cast<float>(p.lnx - 1), static_cast<float>(p.lny - 1), static_cast<float>(p.lnz - 1), u,
+                                   v, w));
+}
+
 // ------------------------------------------------------------- render ----
 struct Cam {
     F3 eye, fwd, right, up;
@@ -115,6 +166,16 @@ __device__ __forceinline__ Cam cam_of(const DevParams& p) {
     c.tanHalfFov = p.tanHalfFov;
     c.aspect = p.aspect;
     return c;
+}
+
+// projectPoint, src/fire_cuda.cu:348-357: frame uv and view depth (z < 0 behind the camera)
+__device__ __forceinline__ F3 project_point(const Cam& cam, F3 point) {
+    const F3 view = sub3(point, cam.eye);
+    const float depth = dot3(view, cam.fwd);
+    if (depth <= 0.02f) return f3(-10.0f, -10.0f, -1.0f);
+    const float px = dot3(view, cam.right) / (depth * cam.tanHalfFov * cam.aspect);
+    const float py = dot3(view, cam.up) / (depth * cam.tanHalfFov);
+    return f3(0.5f + px * 0.5f, 0.5f - py * 0.5f, depth);
 }
 
 __device__ __forceinline__ F3 uv_from_world(F3 q) {
@@ -194,8 +255,8 @@ __device__ void source_model_overlay(float uvx, float uvy, const Cam& cam, const
 
 // roomBackgroundRay src/fire_cuda.cu:1683-1865 (floorGrid :1567-1573 is 0,
 // so the terms it scales are omitted).
-__device__ F3 room_background(const Cam& cam, float uvx, float uvy, F3 rd, float glow, const float4* L, const float* SH, const DevParams& p,
-                              float time) {
+__device__ F3 room_background(const Cam& cam, float uvx, float uvy, F3 rd, float glow, const float4* L, const float* SH, const float* DT,
+                              const DevParams& p, float time) {
     const float roomHalf = 2.55f, ceilingY = 2.18f;
     const bool cinematic = p.cinematic != 0;
     float bestT = 1.0e6f;
@@ -318,6 +379,11 @@ __device__ F3 room_background(const Cam& cam, float uvx, float uvy, F3 rd, float
     const float lift = 0.66f + sstep(0.006f, 0.13f, lum3(indirect)) * 0.72f;
     const float warmth = surface == 1 ? 1.72f : (surface == 2 ? 0.88f : 1.16f);
     color = add3(color, mul3(mul3v(indirect, albedo), giScale * grazingFalloff * contactOcc * lift * warmth));
+    if (p.dirLight) {
+        const float ndl = fmaxf(0.0f, -(normal.x * p.dirLightDir[0] + normal.y * p.dirLightDir[1] + normal.z * p.dirLightDir[2]));
+        const float td = sample_dir_trans(DT, p, vuv.x, vuv.y, vuv.z);
+        color = add3(color, mul3(mul3v(albedo, f3(p.dirLightColor[0], p.dirLightColor[1], p.dirLightColor[2])), ndl * td * kDirDiffuse));
+    }
     const float shadowedSoot = (1.0f - volVis) * sstep(0.18f, 0.82f, 1.0f - sceneShadow);
     color = lerp3(color, f3(0.006f, 0.007f, 0.008f), shadowedSoot * (surface == 2 ? 0.38f : 0.26f));
     const float wallEdge = surface == 3 ? roomHalf - fabsf(hit.z) : roomHalf - fabsf(hit.x);
@@ -325,7 +391,7 @@ __device__ F3 room_background(const Cam& cam, float uvx, float uvy, F3 rd, float
                                                         : fminf(wallEdge, fminf(hit.y, ceilingY - hit.y));
     color = mul3(color, 0.84f + sstep(0.0f, 0.54f, corner) * 0.16f);
     const float gx = floorf(hit.x * 19.0f + hit.z * 13.0f), gy = floorf(hit.y * 15.0f + time * 0.08f);
-    const float grain = (hash2(static_cast<int>(gx), static_cast<int>(gy), p.saltGrain) - 0.5f) * 0.006f;
+    const float grain = (hash2f(gx, gy, static_cast<int>(gx), static_cast<int>(gy), p.saltGrain) - 0.5f) * 0.006f;
     color = add3(color, f3(grain, grain, grain));
     source_model_overlay(uvx, uvy, cam, p, color);
     const float fog = sstep(3.8f, 0.2f, bestT);
@@ -360,7 +426,8 @@ __device__ __forceinline__ F3 debug_heat_color(float value) {
 
 template <bool B>
 __global__ __launch_bounds__(256) void k_render(float4* __restrict__ out, Scalars<B> S, const float4* __restrict__ L, const float* __restrict__ SH,
-                                                const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W, DevParams p) {
+                                                const float* __restrict__ DT, const float* __restrict__ U, const float* __restrict__ V,
+                                                const float* __restrict__ W, DevParams p) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x >= p.fw || y >= p.fh) return;
@@ -382,7 +449,7 @@ __global__ __launch_bounds__(256) void k_render(float4* __restrict__ out, Scalar
         const float t0 = fmaxf(0.0f, boxNear);
         const float t1 = fminf(boxFar, t0 + 3.05f);
         const float stepT = (t1 - t0) / static_cast<float>(steps);
-        const float jitter = hash2(x, y, p.saltJitter);
+        const float jitter = hash2f((static_cast<float>(x) + 0.5f) * 0.754877f, (static_cast<float>(y) + 0.5f) * 0.569840f, x, y, p.saltJitter);
         const float cellX = 1.0f / static_cast<float>(p.nx);
         const float cellY = 1.0f / static_cast<float>(p.ny);
         const float cellZ = 1.0f / static_cast<float>(p.nz);
@@ -550,6 +617,10 @@ __global__ __launch_bounds__(256) void k_render(float4* __restrict__ out, Scalar
             accum = add3(accum, mul3(f3(1.0f, 0.24f, 0.035f), trans * coalGlow * stepT));
             accum = add3(accum, mul3(smokeColor, trans * scatterGain * smokeOnly));
             accum = add3(accum, mul3(smokeLight, trans * sceneScatter * (0.18f + warmDamp * 0.42f)));
+            if (p.dirLight) {
+                const float td = sample_dir_trans(DT, p, wu, wv, ww);
+                accum = add3(accum, mul3(f3(p.dirLightColor[0], p.dirLightColor[1], p.dirLightColor[2]), trans * scatterAlpha * td * kDirScatter));
+            }
             flameGlow = fmaxf(flameGlow, flameDensity * radiantPower * (0.12f + filament * 0.18f + whiteFil * 0.20f));
             smokeOcc = sat(smokeOcc + smokeAlpha * trans);
             trans *= expf(-extinction);
@@ -568,7 +639,7 @@ __global__ __launch_bounds__(256) void k_render(float4* __restrict__ out, Scalar
         return;
     }
     F3 color = accum;
-    if (trans > 0.006f) color = add3(mul3(room_background(cam, uvx, uvy, rd, baseGlow, L, SH, p, t), trans), accum);
+    if (trans > 0.006f) color = add3(mul3(room_background(cam, uvx, uvy, rd, baseGlow, L, SH, DT, p, t), trans), accum);
     const float bloom = sat(flameGlow * 0.10f) * (1.0f - sat(smokeOcc * 0.95f));
     color = add3(color, mul3(f3(1.0f, 0.25f, 0.045f), bloom * bloom * 0.017f * p.reflectionGain));
     out[o] = make_float4(fmaxf(0.0f, color.x), fmaxf(0.0f, color.y), fmaxf(0.0f, color.z), 1.0f);
@@ -577,6 +648,38 @@ __global__ __launch_bounds__(256) void k_render(float4* __restrict__ out, Scalar
 // ---------------------------------------------------------------- K21 ----
 __device__ __forceinline__ float aces(float v) {
     return sat((v * (2.51f * v + 0.03f)) / (v * (2.43f * v + 0.59f) + 0.14f));
+}
+
+// Gizmos (drawGizmos, src/fire_cuda.cu:1909-1951): a ring and dot on the
+// projected source, then the world x/y/z axes from a corner anchor, alpha-
+// blended over the 8-bit colour in frame uv. The reference runs them as a
+// second kernel over the packed frame (overlayKernel :2468-2494, launched
+// after the tonemap at :3419-3425) that unpacks each pixel to value/255; the
+// tonemap below does the same unpack from its own packed bytes, so one pass
+// gives the same bytes without a second read and write of the frame.
+__device__ __forceinline__ float seg_alpha(float px, float py, float ax, float ay, float bx, float by, float thickness) {
+    const float pax = px - ax, pay = py - ay, bax = bx - ax, bay = by - ay;
+    const float h = sat((pax * bax + pay * bay) / fmaxf(0.000001f, bax * bax + bay * bay));
+    const float dx = pax - bax * h, dy = pay - bay * h;
+    return sstep(thickness, thickness * 0.35f, sqrtf(dx * dx + dy * dy));
+}
+__device__ __forceinline__ void draw_gizmos(F3& color, float uvx, float uvy, const Cam& cam, const DevParams& p) {
+    const bool burner = p.sceneId == 2 && p.burnerCount > 0;
+    const F3 src = project_point(cam, f3(burner ? p.burnerX[0] : p.emitterCenterX, burner ? p.burnerY[0] : p.emitterHeightNorm * 2.03f + 0.02f,
+                                         burner ? p.burnerZ[0] : p.emitterCenterZ));
+    if (src.z > 0.0f) {
+        const float dx = uvx - src.x, dy = uvy - src.y;
+        const float r = sqrtf(dx * dx + dy * dy);
+        const float ring = sstep(0.006f, 0.006f * 0.25f, fabsf(r - 0.125f / src.z));
+        const float dot = sstep(0.012f, 0.012f * 0.45f, r);
+        const F3 tint = p.activeGizmo == 1 ? f3(1.0f, 0.34f, 0.055f) : f3(0.45f, 0.18f, 0.055f);
+        color = lerp3(color, tint, sat((ring + dot) * 0.85f));
+    }
+    // axis tips: base + 0.07 * (e . right, -(e . up)) for the unit axes e
+    const float bx = 0.90f, by = 0.84f;
+    color = lerp3(color, f3(1.0f, 0.12f, 0.08f), sat(seg_alpha(uvx, uvy, bx, by, bx + cam.right.x * 0.070f, by - cam.up.x * 0.070f, 0.005f)));
+    color = lerp3(color, f3(0.22f, 1.0f, 0.30f), sat(seg_alpha(uvx, uvy, bx, by, bx + cam.right.y * 0.070f, by - cam.up.y * 0.070f, 0.005f)));
+    color = lerp3(color, f3(0.20f, 0.48f, 1.0f), sat(seg_alpha(uvx, uvy, bx, by, bx + cam.right.z * 0.070f, by - cam.up.z * 0.070f, 0.005f)));
 }
 
 __global__ __launch_bounds__(256) void k_tonemap(uint32_t* __restrict__ out, const float4* __restrict__ hdr, DevParams p) {
@@ -596,11 +699,18 @@ __global__ __launch_bounds__(256) void k_tonemap(uint32_t* __restrict__ out, con
     c = mul3(c, 1.0f + sstep(0.56f, 0.90f, luma) * 0.28f);
     const float g = 1.0f / 2.2f;
     c = f3(powf(sat(c.x), g), powf(sat(c.y), g), powf(sat(c.z), g));
-    const float dither = (hash2(x, y, p.saltDither) - 0.5f) / 255.0f;
+    const float dither = (hash2f(static_cast<float>(x) * 0.754877f, static_cast<float>(y) * 0.569840f, x, y, p.saltDither) - 0.5f) / 255.0f;
     c = add3(c, f3(dither, dither, dither));
-    const uint32_t rr = static_cast<uint32_t>(sat(c.x) * 255.0f);
-    const uint32_t gg = static_cast<uint32_t>(sat(c.y) * 255.0f);
-    const uint32_t bb = static_cast<uint32_t>(sat(c.z) * 255.0f);
+    uint32_t rr = static_cast<uint32_t>(sat(c.x) * 255.0f);
+    uint32_t gg = static_cast<uint32_t>(sat(c.y) * 255.0f);
+    uint32_t bb = static_cast<uint32_t>(sat(c.z) * 255.0f);
+    if (p.showGizmos && p.debugMode == 0) {
+        F3 g8 = f3(static_cast<float>(rr) / 255.0f, static_cast<float>(gg) / 255.0f, static_cast<float>(bb) / 255.0f);
+        draw_gizmos(g8, (static_cast<float>(x) + 0.5f) / static_cast<float>(p.fw), (static_cast<float>(y) + 0.5f) / static_cast<float>(p.fh), cam_of(p), p);
+        rr = static_cast<uint32_t>(sat(g8.x) * 255.0f);
+        gg = static_cast<uint32_t>(sat(g8.y) * 255.0f);
+        bb = static_cast<uint32_t>(sat(g8.z) * 255.0f);
+    }
     out[i] = 0xff000000u | (rr << 16) | (gg << 8) | bb;
 }
 
@@ -611,14 +721,6 @@ __global__ __launch_bounds__(256) void k_tonemap(uint32_t* __restrict__ out, con
 // k_ember_splat has each pixel add the sparks that cover it in spark order, so
 // the frame is deterministic and needs no atomics. The reference's sin-hash
 // per spark (hash21(i, {3.7, 9.1, 19.4, 31.6})) is the integer hash of (i, k).
-__device__ __forceinline__ F3 project_point(const Cam& cam, F3 point) {
-    const F3 view = sub3(point, cam.eye);
-    const float depth = dot3(view, cam.fwd);
-    if (depth <= 0.02f) return f3(-10.0f, -10.0f, -1.0f);
-    const float px = dot3(view, cam.right) / (depth * cam.tanHalfFov * cam.aspect);
-    const float py = dot3(view, cam.up) / (depth * cam.tanHalfFov);
-    return f3(0.5f + px * 0.5f, 0.5f - py * 0.5f, depth);
-}
 
 template <bool B>
 __global__ __launch_bounds__(kMaxEmbers) void k_ember_prepare(Ember* __restrict__ out, Scalars<B> S, const float* __restrict__ U,
@@ -632,7 +734,9 @@ __global__ __launch_bounds__(kMaxEmbers) void k_ember_prepare(Ember* __restrict_
     const int sid = p.sceneId;
     const float baseGlow = sat(sample_scalar<HEAT, B>(S, p, 0.50f, 0.06f, 0.50f) * 0.42f);
     const float sparkGate = sstep(0.02f, 0.30f, baseGlow);
-    const float h0 = hash2(i, 0, p.saltEmber), h1 = hash2(i, 1, p.saltEmber), h2 = hash2(i, 2, p.saltEmber), h3 = hash2(i, 3, p.saltEmber);
+    const float seed = static_cast<float>(i);
+    const float h0 = hash2f(seed, 3.7f, i, 0, p.saltEmber), h1 = hash2f(seed, 9.1f, i, 1, p.saltEmber);
+    const float h2 = hash2f(seed, 19.4f, i, 2, p.saltEmber), h3 = hash2f(seed, 31.6f, i, 3, p.saltEmber);
     bool live = i < count && p.debugMode == 0 && sparkGate > 0.0001f && !(sid == 1 && h2 > 0.34f) && !(sid == 2 && h0 > 0.14f);
     if (live) {
         const float lt = time * (0.20f + h2 * 0.18f + (sid == 2 ? 0.36f : 0.0f)) + h0;
@@ -788,6 +892,7 @@ hipError_t launch_light(DeviceScene& s, const DevParams& p, hipStream_t st) {
     else k_light_build<false><<<g, b, 0, st>>>(s.light, s.shadow, scal<false>(s.s), p);
     k_light_propagate<<<g, b, 0, st>>>(s.lightNext, s.light, s.shadow, p, 0.48f, 0.92f);
     k_light_propagate<<<g, b, 0, st>>>(s.light, s.lightNext, s.shadow, p, 0.34f, 0.96f);
+    if (p.dirLight) k_dir_trans<<<g, b, 0, st>>>(s.dirT, s.light, p);
     return hipGetLastError();
 }
 
@@ -814,6 +919,16 @@ __global__ __launch_bounds__(256) void kb_light_propagate(const BatchTable* __re
     });
 }
 
+// Directional-light transmittance of every member that has the light on.
+__global__ __launch_bounds__(256) void kb_dir_trans(const BatchTable* __restrict__ T) {
+    batch_blocks(T->lightStart, T->count, [&](int s, uint32_t local) {
+        const BatchScene& S = T->scene[s];
+        if (!S.p.dirLight) return;
+        const uint3 b = block_of(local, S.lightG);
+        dir_trans_at(S.dirT, S.light, S.p, b.x * blockDim.x + threadIdx.x, b.y * blockDim.y + threadIdx.y, b.z * blockDim.z + threadIdx.z);
+    });
+}
+
 hipError_t launch_batch_light(const BatchTable* table, bool bf16, int grid, hipStream_t st) {
     const dim3 b = light_block();
     const int g = grid / 4 < 16 ? 16 : grid / 4 - (grid / 4) % 8;  // the light grid is 1/64 of the cells
@@ -821,14 +936,15 @@ hipError_t launch_batch_light(const BatchTable* table, bool bf16, int grid, hipS
     else kb_light_build<false><<<g, b, 0, st>>>(table);
     kb_light_propagate<<<g, b, 0, st>>>(table, 0);
     kb_light_propagate<<<g, b, 0, st>>>(table, 1);
+    kb_dir_trans<<<g, b, 0, st>>>(table);
     return hipGetLastError();
 }
 
 hipError_t launch_render(DeviceScene& s, const DevParams& p, hipStream_t st) {
     const dim3 b(16, 16, 1);
     const dim3 g((s.fw + 15) / 16, (s.fh + 15) / 16, 1);
-    if (s.bf16) k_render<true><<<g, b, 0, st>>>(s.hdr, scal<true>(s.s), s.light, s.shadow, s.u, s.v, s.w, p);
-    else k_render<false><<<g, b, 0, st>>>(s.hdr, scal<false>(s.s), s.light, s.shadow, s.u, s.v, s.w, p);
+    if (s.bf16) k_render<true><<<g, b, 0, st>>>(s.hdr, scal<true>(s.s), s.light, s.shadow, s.dirT, s.u, s.v, s.w, p);
+    else k_render<false><<<g, b, 0, st>>>(s.hdr, scal<false>(s.s), s.light, s.shadow, s.dirT, s.u, s.v, s.w, p);
     if (p.emberCount > 0 && p.debugMode == 0) {
         if (s.bf16) k_ember_prepare<true><<<1, kMaxEmbers, 0, st>>>(s.embers, scal<true>(s.s), s.u, s.v, s.w, p);
         else k_ember_prepare<false><<<1, kMaxEmbers, 0, st>>>(s.embers, scal<false>(s.s), s.u, s.v, s.w, p);

@@ -68,6 +68,16 @@ void clamp_params(const fs_params& in, fs_params& o) {
     o.sor_omega = clampf(in.sor_omega, 0.05f, 1.95f);
     o.sor_iterations = clampi(in.sor_iterations, 1, 1000);
     o.ember_count = clampi(in.ember_count, 0, fs::kMaxEmbers);
+    o.closed_top = in.closed_top != 0 ? 1 : 0;
+    o.noise_forcing = in.noise_forcing != 0 ? 1 : 0;
+    o.curl_noise = clampf(in.curl_noise, 0.0f, 4.0f);
+    o.noise_mode = clampi(in.noise_mode, 0, 1);
+    o.show_gizmos = in.show_gizmos != 0 ? 1 : 0;
+    o.active_gizmo = in.active_gizmo;
+    o.dir_light = in.dir_light != 0 ? 1 : 0;
+    o.dir_light_r = clampf(in.dir_light_r, 0.0f, 20.0f);
+    o.dir_light_g = clampf(in.dir_light_g, 0.0f, 20.0f);
+    o.dir_light_b = clampf(in.dir_light_b, 0.0f, 20.0f);
 }
 
 uint32_t pcg_host(uint32_t v) {
@@ -104,9 +114,12 @@ std::vector<size_t> buffer_sizes(const Dims& d) {
         b.push_back(wc * 4);
     }
     for (int k = 0; k < 6; ++k) b.push_back(c * 4);  // p, p2, d, cu, cv, cw
+    const size_t edges = static_cast<size_t>(d.nx) * (d.ny + 1) * d.nz;
+    for (int k = 0; k < 3; ++k) b.push_back(edges * 4);  // curl-noise potential
     b.push_back(d.lcells() * 16);
     b.push_back(d.lcells() * 16);
     b.push_back(d.lcells() * 4);
+    b.push_back(d.lcells() * 4);  // directional-light transmittance
     b.push_back(d.pixels() * 16);
     b.push_back(d.pixels() * 4);
     b.push_back(d.pixels() * 8);
@@ -222,13 +235,32 @@ DevParams dev_params(const fs_scene& sc) {
     }
     p.omega = s.sor_omega;
     p.iterations = s.sor_iterations;
-    p.saltNoise2 = pcg_host(s.noise_seed + 1u * 0x9E3779B9u);
-    p.saltNoise3 = pcg_host(s.noise_seed + 2u * 0x9E3779B9u);
-    p.saltJitter = pcg_host(s.noise_seed + 3u * 0x9E3779B9u);
-    p.saltDither = pcg_host(s.noise_seed + 4u * 0x9E3779B9u);
-    p.saltGrain = pcg_host(s.noise_seed + 5u * 0x9E3779B9u);
-    p.saltEmber = pcg_host(s.noise_seed + 6u * 0x9E3779B9u);
+    auto key = [&](uint32_t k) { return fs::NoiseKey{pcg_host(s.noise_seed + k * 0x9E3779B9u), s.noise_mode}; };
+    p.saltNoise2 = key(1);
+    p.saltNoise3 = key(2);
+    p.saltJitter = key(3);
+    p.saltDither = key(4);
+    p.saltGrain = key(5);
+    p.saltEmber = key(6);
+    for (int c = 0; c < 3; ++c) p.saltCurl[c] = fs::NoiseKey{pcg_host(s.noise_seed + (7u + c) * 0x9E3779B9u), 0};
     p.emberCount = s.ember_count;
+    p.closedTop = s.closed_top;
+    p.curlOn = s.noise_forcing;
+    p.curlNoise = s.curl_noise;
+    p.showGizmos = s.show_gizmos;
+    p.activeGizmo = s.active_gizmo;
+    p.dirLight = s.dir_light;
+    {
+        const float dx = s.dir_light_x, dy = s.dir_light_y, dz = s.dir_light_z;
+        const float len = std::sqrt(dx * dx + dy * dy + dz * dz);
+        const bool ok = len > 1e-6f && std::isfinite(len);
+        p.dirLightDir[0] = ok ? dx / len : 0.0f;
+        p.dirLightDir[1] = ok ? dy / len : -1.0f;
+        p.dirLightDir[2] = ok ? dz / len : 0.0f;
+    }
+    p.dirLightColor[0] = s.dir_light_r;
+    p.dirLightColor[1] = s.dir_light_g;
+    p.dirLightColor[2] = s.dir_light_b;
     camera_of(s, d.fw, d.fh, p);
     p.emitterCount = static_cast<int>(std::min<size_t>(sc.emitters.size(), fs::kMaxEmitters));
     for (int i = 0; i < p.emitterCount; ++i) {
@@ -364,7 +396,8 @@ bool field_info(const fs_scene* s, int f, size_t& count) {
         case FS_FIELD_PRESSURE:
         case FS_FIELD_DIVERGENCE: count = d.cells(); return true;
         case FS_FIELD_LIGHT: count = 4 * d.lcells(); return true;
-        case FS_FIELD_SHADOW: count = d.lcells(); return true;
+        case FS_FIELD_SHADOW:
+        case FS_FIELD_DIR_TRANSMITTANCE: count = d.lcells(); return true;
         default: return false;
     }
 }
@@ -379,6 +412,7 @@ void* field_ptr(fs_scene* s, int f) {
         case FS_FIELD_DIVERGENCE: return s->ds.d;
         case FS_FIELD_LIGHT: return s->ds.light;
         case FS_FIELD_SHADOW: return s->ds.shadow;
+        case FS_FIELD_DIR_TRANSMITTANCE: return s->ds.dirT;
         default: return nullptr;
     }
 }
@@ -548,8 +582,10 @@ int create_scene(const fs_scene_desc* desc, void* memory, uint64_t memoryBytes, 
     ds.uN = static_cast<float*>(take()); ds.vN = static_cast<float*>(take()); ds.wN = static_cast<float*>(take());
     ds.p = static_cast<float*>(take()); ds.p2 = static_cast<float*>(take()); ds.d = static_cast<float*>(take());
     ds.cu = static_cast<float*>(take()); ds.cv = static_cast<float*>(take()); ds.cw = static_cast<float*>(take());
+    for (int k = 0; k < 3; ++k) ds.psi[k] = static_cast<float*>(take());
     ds.light = static_cast<float4*>(take()); ds.lightNext = static_cast<float4*>(take());
     ds.shadow = static_cast<float*>(take());
+    ds.dirT = static_cast<float*>(take());
     ds.hdr = static_cast<float4*>(take());
     ds.bgra = static_cast<uint32_t*>(take());
     ds.rgba16 = static_cast<uint16_t*>(take());

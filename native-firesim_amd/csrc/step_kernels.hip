@@ -95,7 +95,7 @@ __device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float
     }
     if (x < p.nx && z < p.nz) {
         float out = 0.0f;
-        if (y != 0) {
+        if (y != 0 && !(p.closedTop && y == p.ny)) {
             const F3 vel = vel_from_axes(U, V, W, p, cxF, cxC, ayF, ayC, czF, czC);
             out = sample_v<double>(V, p, cx - vel.x * k, ay - vel.y * k, cz - vel.z * k) * 0.996f;
         }
@@ -362,20 +362,68 @@ __global__ __launch_bounds__(256) void k_finalize_release(const double* __restri
 }
 
 // ---------------------------------------------------------- K9-K11 ----
+// Divergence-free curl-noise forcing (an extension that replaces the fbm
+// noise terms of src/fire_cuda.cu:1164-1165 when curlOn is set). A vector
+// potential psi lives on the grid edges of index space: psi_x at
+// (i + 1/2, j, k), psi_y at (i, j + 1/2, k), psi_z at (i, j, k + 1/2). Each
+// face receives curlNoise * dt times the index-space curl of psi around it
+// (u: dpsi_z/dj - dpsi_y/dk, v: dpsi_x/dk - dpsi_z/di, w: dpsi_y/di -
+// dpsi_x/dj), whose discrete divergence, uR - uL + vU - vD + wF - wB, cancels
+// term by term. The tangential components vanish on the walls (and on the top
+// when it is closed), so no flux crosses them. Each component is one octave of
+// integer-hash value noise of the edge's normalized position, drifting with
+// time, scaled so that its curl is of order 1.
+constexpr float kCurlFrequency = 6.0f;  // lattice cells across the domain
+__device__ __forceinline__ float curl_potential(int c, float ex, float ey, float ez, float t, const DevParams& p) {
+    const float scale = static_cast<float>(p.nx + p.ny + p.nz) / (3.0f * kCurlFrequency);
+    return scale * (vnoise3(ex * kCurlFrequency + t * 0.23f, ey * kCurlFrequency - t * 0.17f, ez * kCurlFrequency + t * 0.19f, p.saltCurl[c]) - 0.5f);
+}
+// Arrays of nx * (ny + 1) * nz edges, x fastest; edges at i = nx or k = nz are
+// wall edges (0) and are not stored.
+__device__ __forceinline__ Idx psi_index(const DevParams& p, int i, int j, int k) {
+    return (static_cast<Idx>(k) * static_cast<Idx>(p.ny + 1) + static_cast<Idx>(j)) * static_cast<Idx>(p.nx) + static_cast<Idx>(i);
+}
+__device__ __forceinline__ float psi_at(const float* __restrict__ psi, const DevParams& p, int i, int j, int k) {
+    return (i >= p.nx || k >= p.nz) ? 0.0f : ld_at(psi, psi_index(p, i, j, k));
+}
+// Cell (x, y, z) writes the three edges it owns; the top row of cells also
+// writes the edges of the top face (j = ny).
+__device__ __forceinline__ void write_curl_potential(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz, const DevParams& p, int x,
+                                                     int y, int z, float t) {
+    const float fx = static_cast<float>(x) / static_cast<float>(p.nx), cx = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
+    const float fy = static_cast<float>(y) / static_cast<float>(p.ny), cy = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
+    const float fz = static_cast<float>(z) / static_cast<float>(p.nz), cz = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const bool topWall = p.closedTop != 0;
+    st_at(px, psi_index(p, x, y, z), (y == 0 || z == 0) ? 0.0f : curl_potential(0, cx, fy, fz, t, p));
+    st_at(py, psi_index(p, x, y, z), (x == 0 || z == 0) ? 0.0f : curl_potential(1, fx, cy, fz, t, p));
+    st_at(pz, psi_index(p, x, y, z), (x == 0 || y == 0) ? 0.0f : curl_potential(2, fx, fy, cz, t, p));
+    if (y == p.ny - 1) {
+        st_at(px, psi_index(p, x, p.ny, z), (topWall || z == 0) ? 0.0f : curl_potential(0, cx, 1.0f, fz, t, p));
+        st_at(pz, psi_index(p, x, p.ny, z), (topWall || x == 0) ? 0.0f : curl_potential(2, fx, 1.0f, cz, t, p));
+    }
+}
+
 template <bool B>
-__device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, const Scalars<B>& S,
+__device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, float* __restrict__ psix,
+                                                 float* __restrict__ psiy, float* __restrict__ psiz, const Scalars<B>& S,
                                                  const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                  const DevParams& p, int x, int y, int z) {
     // The two noise fields first, by every lane of the wavefront (lanes are
     // consecutive x cells of one row, so noiseA's lattice row and noiseB's
-    // lattice column are shared by the whole wavefront).
+    // lattice column are shared by the whole wavefront). With curl noise on,
+    // the fbm terms are replaced by the potential's curl on the faces.
     const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
     const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
     const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
     const float t = ld_at(p.clock, 0u);
-    const float noiseA = fbm2_t<1>(u * 9.0f + w * 3.7f + t * 0.22f, v * 13.0f - t * 0.31f, p.saltNoise2) - 0.5f;
-    const float noiseB = fbm2_t<2>(w * 10.0f - t * 0.19f, u * 8.0f + v * 5.5f + t * 0.27f, p.saltNoise2) - 0.5f;
+    const bool curl = p.curlOn;
+    float noiseA = 0.0f, noiseB = 0.0f;
+    if (!curl) {
+        noiseA = fbm2_t<1>(u * 9.0f + w * 3.7f + t * 0.22f, v * 13.0f - t * 0.31f, p.saltNoise2) - 0.5f;
+        noiseB = fbm2_t<2>(w * 10.0f - t * 0.19f, u * 8.0f + v * 5.5f + t * 0.27f, p.saltNoise2) - 0.5f;
+    }
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
+    if (curl) write_curl_potential(psix, psiy, psiz, p, x, y, z, t);
     const Idx idx = sidx(p, x, y, z);
     const int sid = p.sceneId;
     const float h = S.template ld<HEAT>(idx), s = S.template ld<SOOT>(idx), k = S.template ld<TURB>(idx), c = S.template ld<PROGRESS>(idx);
@@ -412,7 +460,8 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
 // partials as k_finalize_release does (same order, same result), hidden
 // behind the other workgroups of this launch.
 template <bool B>
-__global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, Scalars<B> S,
+__global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, float* __restrict__ psix,
+                                                       float* __restrict__ psiy, float* __restrict__ psiz, Scalars<B> S,
                                                        const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                        DevParams p, const double* __restrict__ partial, size_t npartial,
                                                        double* __restrict__ acc) {
@@ -425,7 +474,7 @@ __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, f
         }
     }
     const int3 c = cell_of(xcd_block());
-    force_contrib_at<B>(cu, cv, cw, S, U, V, W, p, c.x, c.y, c.z);
+    force_contrib_at<B>(cu, cv, cw, psix, psiy, psiz, S, U, V, W, p, c.x, c.y, c.z);
 }
 
 // New face values after the gathered force and the boundary clamp
@@ -434,35 +483,57 @@ __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, f
 // formed in FP64 and rounded once when stored; the divergence below uses the
 // unrounded values, since it is a difference of nearly equal face values whose
 // FP32 rounding would dominate its relative error.
-__device__ __forceinline__ double face_u(const float* U, const float* cu, const DevParams& p, int fx, int y, int z) {
+// The potential psi's curl on each face (zero without curl noise), times dt
+// and the amplitude.
+struct CurlPsi {
+    const float *x, *y, *z;
+};
+__device__ __forceinline__ float curl_u(const CurlPsi& s, const DevParams& p, int fx, int y, int z) {
+    const float c = (psi_at(s.z, p, fx, y + 1, z) - psi_at(s.z, p, fx, y, z)) - (psi_at(s.y, p, fx, y, z + 1) - psi_at(s.y, p, fx, y, z));
+    return p.dt * p.curlNoise * c;
+}
+__device__ __forceinline__ float curl_v(const CurlPsi& s, const DevParams& p, int x, int fy, int z) {
+    const float c = (psi_at(s.x, p, x, fy, z + 1) - psi_at(s.x, p, x, fy, z)) - (psi_at(s.z, p, x + 1, fy, z) - psi_at(s.z, p, x, fy, z));
+    return p.dt * p.curlNoise * c;
+}
+__device__ __forceinline__ float curl_w(const CurlPsi& s, const DevParams& p, int x, int y, int fz) {
+    const float c = (psi_at(s.y, p, x + 1, y, fz) - psi_at(s.y, p, x, y, fz)) - (psi_at(s.x, p, x, y + 1, fz) - psi_at(s.x, p, x, y, fz));
+    return p.dt * p.curlNoise * c;
+}
+
+__device__ __forceinline__ double face_u(const float* U, const float* cu, const CurlPsi& psi, const DevParams& p, int fx, int y, int z) {
     if (fx == 0 || fx == p.nx) return 0.0;
     double val = ld_at(U, uidx(p, fx, y, z));
     val += ld_at(cu, sidx(p, fx - 1, y, z));
     val += ld_at(cu, sidx(p, fx, y, z));
+    if (p.curlOn) val += curl_u(psi, p, fx, y, z);
     return fmin(4.0, fmax(-4.0, val));
 }
-__device__ __forceinline__ double face_v(const float* V, const float* cv, const DevParams& p, int x, int fy, int z) {
-    if (fy == 0) return 0.0;
+__device__ __forceinline__ double face_v(const float* V, const float* cv, const CurlPsi& psi, const DevParams& p, int x, int fy, int z) {
+    if (fy == 0 || (p.closedTop && fy == p.ny)) return 0.0;
     double val = ld_at(V, vidx(p, x, fy, z));
     val += ld_at(cv, sidx(p, x, fy - 1, z));
+    if (p.curlOn) val += curl_v(psi, p, x, fy, z);
     return fmin(static_cast<double>(6.4f), fmax(static_cast<double>(-0.9f), val));  // the reference's float bounds
 }
-__device__ __forceinline__ double face_w(const float* W, const float* cw, const DevParams& p, int x, int y, int fz) {
+__device__ __forceinline__ double face_w(const float* W, const float* cw, const CurlPsi& psi, const DevParams& p, int x, int y, int fz) {
     if (fz == 0 || fz == p.nz) return 0.0;
     double val = ld_at(W, widx(p, x, y, fz));
     val += ld_at(cw, sidx(p, x, y, fz - 1));
     val += ld_at(cw, sidx(p, x, y, fz));
+    if (p.curlOn) val += curl_w(psi, p, x, y, fz);
     return fmin(4.0, fmax(-4.0, val));
 }
 
 __device__ __forceinline__ void force_apply_at(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
                                                const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
-                                               float* __restrict__ D, float* __restrict__ P, const DevParams& p, int x, int y, int z) {
+                                               const CurlPsi& psi, float* __restrict__ D, float* __restrict__ P, const DevParams& p, int x, int y,
+                                               int z) {
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
-    const double uL = face_u(U, cu, p, x, y, z), uR = face_u(U, cu, p, x + 1, y, z);
-    const double vD = face_v(V, cv, p, x, y, z), vU = face_v(V, cv, p, x, y + 1, z);
-    const double wB = face_w(W, cw, p, x, y, z), wF = face_w(W, cw, p, x, y, z + 1);
+    const double uL = face_u(U, cu, psi, p, x, y, z), uR = face_u(U, cu, psi, p, x + 1, y, z);
+    const double vD = face_v(V, cv, psi, p, x, y, z), vU = face_v(V, cv, psi, p, x, y + 1, z);
+    const double wB = face_w(W, cw, psi, p, x, y, z), wF = face_w(W, cw, psi, p, x, y, z + 1);
     st_at(uo, uidx(p, x, y, z), static_cast<float>(uL));
     if (x == p.nx - 1) st_at(uo, uidx(p, x + 1, y, z), static_cast<float>(uR));
     st_at(vo, vidx(p, x, y, z), static_cast<float>(vD));
@@ -478,9 +549,9 @@ __device__ __forceinline__ void force_apply_at(float* __restrict__ uo, float* __
 __global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
                                                      const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                      const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
-                                                     float* __restrict__ D, float* __restrict__ P, DevParams p) {
+                                                     CurlPsi psi, float* __restrict__ D, float* __restrict__ P, DevParams p) {
     const int3 c = cell_of(xcd_block());
-    force_apply_at(uo, vo, wo, U, V, W, cu, cv, cw, D, P, p, c.x, c.y, c.z);
+    force_apply_at(uo, vo, wo, U, V, W, cu, cv, cw, psi, D, P, p, c.x, c.y, c.z);
 }
 
 // -------------------------------------------------------------- K12 ----
@@ -889,7 +960,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
         const int3 c = cell_of(block_of(local, S.cellG));
-        force_contrib_at<B>(S.cu, S.cv, S.cw, scal_of<B>(S.sOut), S.velB[0], S.velB[1], S.velB[2], S.p, c.x, c.y, c.z);
+        force_contrib_at<B>(S.cu, S.cv, S.cw, S.psi[0], S.psi[1], S.psi[2], scal_of<B>(S.sOut), S.velB[0], S.velB[1], S.velB[2], S.p, c.x, c.y, c.z);
     });
 }
 
@@ -897,7 +968,8 @@ __global__ __launch_bounds__(256) void kb_force_apply(const BatchTable* __restri
     batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
         const int3 c = cell_of(block_of(local, S.cellG));
-        force_apply_at(S.velA[0], S.velA[1], S.velA[2], S.velB[0], S.velB[1], S.velB[2], S.cu, S.cv, S.cw, S.d, S.p0, S.p, c.x, c.y, c.z);
+        force_apply_at(S.velA[0], S.velA[1], S.velA[2], S.velB[0], S.velB[1], S.velB[2], S.cu, S.cv, S.cw, CurlPsi{S.psi[0], S.psi[1], S.psi[2]}, S.d,
+                       S.p0, S.p, c.x, c.y, c.z);
     });
 }
 
@@ -974,9 +1046,9 @@ hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool
     const dim3 g = cell_grid(s.nx, s.ny, s.nz), b = cell_block();
     const double* part = finalizeRelease ? s.partials : nullptr;
     const size_t np = react_partial_count(s.nx, s.ny, s.nz);
-    if (s.bf16) k_force_contrib<true><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, scal<true>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
-    else k_force_contrib<false><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, scal<false>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
-    k_force_apply<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, s.cu, s.cv, s.cw, s.d, s.p, p);
+    if (s.bf16) k_force_contrib<true><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, s.psi[0], s.psi[1], s.psi[2], scal<true>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
+    else k_force_contrib<false><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, s.psi[0], s.psi[1], s.psi[2], scal<false>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
+    k_force_apply<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, s.cu, s.cv, s.cw, CurlPsi{s.psi[0], s.psi[1], s.psi[2]}, s.d, s.p, p);
     std::swap(s.u, s.uN);
     std::swap(s.v, s.vN);
     std::swap(s.w, s.wN);

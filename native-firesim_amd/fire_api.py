@@ -148,6 +148,8 @@ def settings_to_params(s: FireSettings) -> firesim.Params:
     p.raymarch_steps = c(s.raymarchSteps, 24, 120)
     p.ember_count = c(s.emberCount, 0, 192)
     p.render_debug_mode = c(s.renderDebugMode, 0, 6)
+    p.show_gizmos = 1 if s.showGizmos != 0 else 0
+    p.active_gizmo = s.activeGizmo
     p.exposure, p.reflection_gain = c(s.exposure, 0.45, 2.4), c(s.reflectionGain, 0.0, 2.2)
     p.smoke_darkness = c(s.smokeDarkness, 0.35, 2.4)
     p.emitter_center_x, p.emitter_center_z = s.emitterCenterX, s.emitterCenterZ

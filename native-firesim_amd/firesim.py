@@ -22,9 +22,12 @@ FS_OK = 0
 FIELD_NAMES = [
     "heat", "fuel", "oxygen", "soot", "char", "ash", "pyrolysis", "progress",
     "turbulence", "soot_optics", "u", "v", "w", "pressure", "divergence", "light", "shadow",
+    "dir_transmittance",
 ]
 FIELD = {name: i for i, name in enumerate(FIELD_NAMES)}
 SCALAR_FIELDS = FIELD_NAMES[:10]
+# the simulation state and its light volumes; dir_transmittance is written only while dir_light is on
+STATE_FIELDS = FIELD_NAMES[:17]
 
 # enum fs_stage
 STAGE_ADVECT_VELOCITY, STAGE_ADVECT_REACT, STAGE_FORCE, STAGE_PRESSURE, STAGE_PROJECT, STAGE_LIGHT = range(6)
@@ -64,6 +67,10 @@ class Params(Structure):
         ("sor_omega", c_float), ("sor_iterations", c_int32),
         ("noise_seed", c_uint32),
         ("ember_count", c_int32),
+        ("closed_top", c_int32), ("noise_forcing", c_int32), ("curl_noise", c_float), ("noise_mode", c_int32),
+        ("show_gizmos", c_int32), ("active_gizmo", c_int32),
+        ("dir_light", c_int32), ("dir_light_x", c_float), ("dir_light_y", c_float), ("dir_light_z", c_float),
+        ("dir_light_r", c_float), ("dir_light_g", c_float), ("dir_light_b", c_float),
     ]
 
 
@@ -210,7 +217,9 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> fl
 
 
 def field_shape(field: int, nx: int, ny: int, nz: int) -> tuple:
-    """Numpy shape (z-major, x fastest) of a field."""
+    """Numpy shape (z-major, x fastest) of a field (id or name)."""
+    if isinstance(field, str):
+        field = FIELD[field]
     if field < 10 or field in (FIELD["pressure"], FIELD["divergence"]):
         return (nz, ny, nx)
     if field == FIELD["u"]:
@@ -222,7 +231,7 @@ def field_shape(field: int, nx: int, ny: int, nz: int) -> tuple:
     lnx, lny, lnz = light_dims(nx, ny, nz)
     if field == FIELD["light"]:
         return (lnz, lny, lnx, 4)
-    if field == FIELD["shadow"]:
+    if field in (FIELD["shadow"], FIELD["dir_transmittance"]):
         return (lnz, lny, lnx)
     raise ValueError(field)
 

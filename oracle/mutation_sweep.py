@@ -3,15 +3,17 @@
 
 TEST INFRASTRUCTURE ONLY. Plants one mutation at a time in oracle.cpp -- a
 float literal of the chosen functions scaled by 1.1 -- builds the mutant into
-oracle/liboracle.so, runs the oracle pin tests (tests/test_oracle_pins.py and
-tests/test_oracle.py) and reports the mutants no test fails ("survivors").
-The good library is restored afterwards.
+a temporary library, runs the oracle pin tests against it (ORACLE_LIB) and
+reports the mutants no test fails ("survivors"). oracle/liboracle.so is not
+touched.
 
-    python oracle/mutation_sweep.py --seed 7 --count 120 [--set step|render]
+    python oracle/mutation_sweep.py --seed 7 --count 120 [--set step|render|extensions]
 
 `step` mutates advect-react, force, light, metrics, beds, MacCormack, the
 room colour and its indirect light; `render` the raymarch, embers, tonemap,
-reset, velocity advection, projection and the source-model overlay.
+reset, velocity advection, projection and the source-model overlay;
+`extensions` the curl-noise potential, the directional-light transmittance
+and its use in the render, and the gizmos.
 """
 import argparse
 import os
@@ -25,7 +27,6 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "oracle.cpp")
-LIB = os.path.join(HERE, "liboracle.so")
 
 SETS = {
     "step": [
@@ -48,7 +49,13 @@ SETS = {
         ("project", r"^static void project", r"^static V3 blackbody"),
         ("overlay", r"^static void source_model_overlay", r"^static V3 room_background"),
     ],
+    "extensions": [
+        ("curl", r"^static R curl_edge_potential", r"^// -+ force \+ boundary"),
+        ("dir_trans", r"^static const R kDirKappa", r"^// Build, then two propagations"),
+        ("gizmos", r"^static inline R segment_alpha", r"^static void tonemap"),
+    ],
 }
+TESTS = ["tests/test_oracle_pins.py", "tests/test_oracle.py", "tests/test_oracle_extensions.py"]
 
 
 def candidates(lines, spans):
@@ -75,8 +82,8 @@ def main():
     random.seed(args.seed)
     pick = random.sample(cands, min(args.count, len(cands)))
     tmp = tempfile.mkdtemp(prefix="oracle_mutants_")
-    good = os.path.join(tmp, "liboracle_good.so")
-    shutil.copy(LIB, good)
+    lib = os.path.join(tmp, "liboracle_mutant.so")
+    env = dict(os.environ, ORACLE_LIB=lib)
     survivors = []
     try:
         for k, (name, i, a, b, val) in enumerate(pick):
@@ -86,19 +93,18 @@ def main():
             src = os.path.join(tmp, "oracle_mutant.cpp")
             text = "\n".join(mutated).replace('#include "../include/firesim.h"', f'#include "{os.path.join(REPO, "include", "firesim.h")}"')
             open(src, "w").write(text)
-            r = subprocess.run(["g++", "-O1", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-std=c++17", "-shared", "-o", LIB, src],
+            r = subprocess.run(["g++", "-O1", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-std=c++17", "-shared", "-o", lib, src],
                                capture_output=True)
             if r.returncode:
                 print(f"{k:3d} {name} line {i + 1}: build failed", flush=True)
                 continue
-            t = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", "tests/test_oracle_pins.py",
-                                "tests/test_oracle.py"], cwd=REPO, capture_output=True, text=True, timeout=900)
+            t = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", *TESTS], cwd=REPO, env=env,
+                               capture_output=True, text=True, timeout=900)
             killed = t.returncode != 0
             print(f"{k:3d} {name:16s} line {i + 1:5d} {val:>10s} -> {new:>22s}  {'killed' if killed else 'SURVIVED'}", flush=True)
             if not killed:
                 survivors.append((name, i + 1, val, lines[i].strip()))
     finally:
-        shutil.copy(good, LIB)
         shutil.rmtree(tmp, ignore_errors=True)
     print(f"survivors: {len(survivors)} of {len(pick)}")
     for s in survivors:

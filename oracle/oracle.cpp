@@ -215,6 +215,17 @@ static void clamp_params(const fs_params& in, fs_params& o) {
     }
     o.sor_omega = clampf(in.sor_omega, 0.05f, 1.95f);
     o.sor_iterations = clampi(in.sor_iterations, 1, 1000);
+    // extensions (no reference counterpart)
+    o.closed_top = in.closed_top != 0 ? 1 : 0;
+    o.noise_forcing = in.noise_forcing != 0 ? 1 : 0;
+    o.curl_noise = clampf(in.curl_noise, 0.0f, 4.0f);
+    o.noise_mode = clampi(in.noise_mode, 0, 1);
+    o.show_gizmos = in.show_gizmos != 0 ? 1 : 0;
+    o.active_gizmo = in.active_gizmo;
+    o.dir_light = in.dir_light != 0 ? 1 : 0;
+    o.dir_light_r = clampf(in.dir_light_r, 0.0f, 20.0f);
+    o.dir_light_g = clampf(in.dir_light_g, 0.0f, 20.0f);
+    o.dir_light_b = clampf(in.dir_light_b, 0.0f, 20.0f);
 }
 
 struct Cam {
@@ -237,7 +248,14 @@ struct P {
     R omega;
     int iterations;
     uint32_t saltNoise2, saltNoise3, saltJitter, saltDither, saltGrain, saltEmber;
+    uint32_t saltCurl[3];
     int emberCount;
+    int closedTop;
+    int curlOn;
+    R curlNoise;
+    int showGizmos, activeGizmo;
+    int dirLight;
+    R dirLightDir[3], dirLightColor[3];
     Cam cam;
     int emitterCount;
     R ex[FS_MAX_EMITTERS], ey[FS_MAX_EMITTERS], ez[FS_MAX_EMITTERS], er[FS_MAX_EMITTERS];
@@ -268,6 +286,7 @@ struct State {
     std::vector<R> u, v, w, p, d;
     std::vector<R> light;   // 4 per light cell
     std::vector<R> shadow;  // 1 per light cell
+    std::vector<R> dirT;    // directional-light transmittance, 1 per light cell
     float time = 0.0f, renderTime = 0.0f;
     int frameIndex = 0;
     R heatReleaseStep = 0.0, heatReleaseTotal = 0.0;
@@ -325,6 +344,25 @@ static P make_p(const State& st, R time) {
     p.saltDither = salt_of(s.noise_seed, kSaltDither);
     p.saltGrain = salt_of(s.noise_seed, kSaltGrain);
     p.saltEmber = salt_of(s.noise_seed, kSaltEmber);
+    for (int c = 0; c < 3; ++c) p.saltCurl[c] = salt_of(s.noise_seed, 7u + c);
+    p.closedTop = s.closed_top;
+    p.curlOn = s.noise_forcing;
+    p.curlNoise = s.curl_noise;
+    p.showGizmos = s.show_gizmos;
+    p.activeGizmo = s.active_gizmo;
+    p.dirLight = s.dir_light;
+    {
+        // the light's direction in FP32, normalized as the product normalizes it
+        const float dx = s.dir_light_x, dy = s.dir_light_y, dz = s.dir_light_z;
+        const float len = std::sqrt(dx * dx + dy * dy + dz * dz);
+        const bool ok = len > 1e-6f && std::isfinite(len);
+        p.dirLightDir[0] = ok ? dx / len : 0.0f;
+        p.dirLightDir[1] = ok ? dy / len : -1.0f;
+        p.dirLightDir[2] = ok ? dz / len : 0.0f;
+    }
+    p.dirLightColor[0] = s.dir_light_r;
+    p.dirLightColor[1] = s.dir_light_g;
+    p.dirLightColor[2] = s.dir_light_b;
     p.emberCount = s.ember_count;
     p.cam = make_camera(s, st.fw, st.fh);
     p.emitterCount = static_cast<int>(std::min<size_t>(st.emitters.size(), FS_MAX_EMITTERS));
@@ -575,6 +613,7 @@ static void reset(State& st) {
     std::fill(st.w.begin(), st.w.end(), 0.0);
     std::fill(st.light.begin(), st.light.end(), 0.0);
     std::fill(st.shadow.begin(), st.shadow.end(), 0.0);
+    std::fill(st.dirT.begin(), st.dirT.end(), 0.0);
     st.time = 0.0f;
     st.renderTime = 0.0f;
     st.frameIndex = 0;
@@ -605,7 +644,7 @@ static void advect_velocity(State& st, const P& p) {
                 }
                 if (x < st.nx && z < st.nz) {  // v face
                     R out = 0.0;
-                    if (y != 0) {
+                    if (y != 0 && !(p.closedTop && y == st.ny)) {
                         const R fu = (x + 0.5) / st.nx, fv = static_cast<R>(y) / st.ny, fw = (z + 0.5) / st.nz;
                         const V3 vel = sample_velocity(st, U, Vv, W, fu, fv, fw);
                         out = sample_v(st, Vv, fu - vel.x * p.dt * k, fv - vel.y * p.dt * k, fw - vel.z * p.dt * k) * 0.996f;
@@ -806,6 +845,29 @@ static void advect_react(State& st, const P& p) {
     st.heatReleaseTotal += heatRelease;
 }
 
+// ----------------------------------------------------- curl-noise forcing ----
+// Extension replacing the fbm noise terms (:1164-1165) when noise_forcing is 1: a
+// vector potential on the grid edges of index space (psi_x at (i + 1/2, j, k),
+// psi_y at (i, j + 1/2, k), psi_z at (i, j, k + 1/2)), one octave of value noise
+// of the edge's normalized position drifting with time, scaled by
+// (nx + ny + nz) / (3 * 6); its tangential components are zero on the walls
+// (and the top when closed). Each face receives dt * amplitude times the
+// index-space curl, whose discrete divergence cancels term by term.
+static R curl_edge_potential(const State& st, const P& p, int c, int i, int j, int k) {
+    const bool topWall = p.closedTop != 0;
+    // walls: x = 0 and nx, z = 0 and nz, the floor, and the top if closed
+    if (c == 0 && (j == 0 || (topWall && j == st.ny) || k == 0 || k == st.nz || i >= st.nx)) return 0.0;
+    if (c == 1 && (i == 0 || i == st.nx || k == 0 || k == st.nz || j >= st.ny)) return 0.0;
+    if (c == 2 && (i == 0 || i == st.nx || j == 0 || (topWall && j == st.ny) || k >= st.nz)) return 0.0;
+    const R ex = (c == 0 ? i + 0.5 : static_cast<R>(i)) / st.nx;
+    const R ey = (c == 1 ? j + 0.5 : static_cast<R>(j)) / st.ny;
+    const R ez = (c == 2 ? k + 0.5 : static_cast<R>(k)) / st.nz;
+    const R F = 6.0f;
+    const R scale = static_cast<float>(st.nx + st.ny + st.nz) / (3.0f * 6.0f);
+    const R t = p.time;
+    return scale * (value_noise3(ex * F + t * 0.23f, ey * F - t * 0.17f, ez * F + t * 0.19f, p.saltCurl[c]) - 0.5);
+}
+
 // ------------------------------------------- force + boundary + divergence ----
 // forceVelocityKernel src/fire_cuda.cu:1128-1183 in gather form, then
 // velocityBoundaryKernel :1261-1291, then divergenceKernel :1185-1199.
@@ -837,8 +899,10 @@ static void force_boundary_divergence(State& st, const P& p) {
                 const R curlY = (uR - uL) - (wF - wB);
                 const R curlZ = (vU - vD) - (uR - uL);
                 const R curlMag = std::fabs(curlX) + std::fabs(curlY) + std::fabs(curlZ);
-                const R noiseA = fbm2(u * 9.0f + w * 3.7f + p.time * 0.22f, v * 13.0f - p.time * 0.31f, p.saltNoise2) - 0.5f;
-                const R noiseB = fbm2(w * 10.0f - p.time * 0.19f, u * 8.0f + v * 5.5f + p.time * 0.27f, p.saltNoise2) - 0.5f;
+                // with curl noise on, the fbm terms are replaced by the potential's curl on the faces
+                const bool curl = p.curlOn != 0;
+                const R noiseA = curl ? 0.0 : fbm2(u * 9.0f + w * 3.7f + p.time * 0.22f, v * 13.0f - p.time * 0.31f, p.saltNoise2) - 0.5f;
+                const R noiseB = curl ? 0.0 : fbm2(w * 10.0f - p.time * 0.19f, u * 8.0f + v * 5.5f + p.time * 0.27f, p.saltNoise2) - 0.5f;
                 const R thermal = smoothstep(0.04f, 4.20f, h + s * 0.28f);
                 const R upperDrag = smoothstep(0.26f, 0.94f, v);
                 const R lesGain = std::sqrt(std::max<R>(0.0, k)) * (0.30f + c * 0.54f);
@@ -857,6 +921,10 @@ static void force_boundary_divergence(State& st, const P& p) {
             }
         }
     }
+    // curl-noise potential on the grid edges (zero on the walls)
+    const bool curl = p.curlOn != 0;
+    auto psi = [&](int c, int i, int j, int k) -> R { return curl_edge_potential(st, p, c, i, j, k); };
+    const R kc = p.dt * p.curlNoise;
     // gather onto faces, then the boundary clamp (K10)
 #pragma omp parallel for schedule(static)
     for (int z = 0; z <= st.nz; ++z) {
@@ -867,19 +935,23 @@ static void force_boundary_divergence(State& st, const P& p) {
                     R val = st.u[f];
                     if (x - 1 >= 0) val += cu[st.si(x - 1, y, z)];
                     if (x < st.nx) val += cu[st.si(x, y, z)];
+                    if (curl) val += kc * ((psi(2, x, y + 1, z) - psi(2, x, y, z)) - (psi(1, x, y, z + 1) - psi(1, x, y, z)));
                     st.u[f] = (x == 0 || x == st.nx) ? 0.0 : std::min<R>(4.0f, std::max<R>(-4.0f, val));
                 }
                 if (x < st.nx && z < st.nz) {
                     const size_t f = st.vi(x, y, z);
                     R val = st.v[f];
                     if (y - 1 >= 0) val += cv[st.si(x, y - 1, z)];
-                    st.v[f] = (y == 0) ? 0.0 : std::min<R>(6.4f, std::max<R>(-0.9f, val));
+                    if (curl) val += kc * ((psi(0, x, y, z + 1) - psi(0, x, y, z)) - (psi(2, x + 1, y, z) - psi(2, x, y, z)));
+                    const bool wall = y == 0 || (p.closedTop && y == st.ny);
+                    st.v[f] = wall ? 0.0 : std::min<R>(6.4f, std::max<R>(-0.9f, val));
                 }
                 if (x < st.nx && y < st.ny) {
                     const size_t f = st.wi(x, y, z);
                     R val = st.w[f];
                     if (z - 1 >= 0) val += cw[st.si(x, y, z - 1)];
                     if (z < st.nz) val += cw[st.si(x, y, z)];
+                    if (curl) val += kc * ((psi(1, x + 1, y, z) - psi(1, x, y, z)) - (psi(0, x, y + 1, z) - psi(0, x, y, z)));
                     st.w[f] = (z == 0 || z == st.nz) ? 0.0 : std::min<R>(4.0f, std::max<R>(-4.0f, val));
                 }
             }
@@ -1090,6 +1162,47 @@ static void light_propagate(const State& st, const std::vector<R>& in, std::vect
     }
 }
 
+// Directional light (an extension, no reference counterpart): Beer-Lambert
+// transmittance toward the light from every light-cell centre. Samples at
+// s_k = (k + 1/2) ds with weights sat(1/2 + (len - s_k)/ds), len = the ray's
+// length inside the volume box (slab exits); sum_k ds w_k = len exactly.
+static const R kDirKappa = 3.0f, kDirScatter = 2.0f, kDirDiffuse = 0.10f;
+static inline R dir_step(const State& st) {
+    return 0.5f * std::min(std::min(static_cast<R>(2.10f) / st.lnx, static_cast<R>(2.03f) / st.lny), static_cast<R>(1.64f) / st.lnz);
+}
+static inline R slab_exit(R x0, R d, R lo, R hi) {
+    if (d > 1.0e-6f) return (hi - x0) / d;
+    if (d < -1.0e-6f) return (lo - x0) / d;
+    return 1.0e30f;
+}
+static void dir_transmittance(State& st, const P& p) {
+    const R ds = dir_step(st);
+    const R dx = -p.dirLightDir[0], dy = -p.dirLightDir[1], dz = -p.dirLightDir[2];
+    std::vector<R> ext(st.lcells());
+    for (size_t i = 0; i < ext.size(); ++i) ext[i] = st.light[4 * i + 3];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int z = 0; z < st.lnz; ++z)
+        for (int y = 0; y < st.lny; ++y)
+            for (int x = 0; x < st.lnx; ++x) {
+                const R x0 = (x + 0.5) / st.lnx * 2.10f - 1.05f;
+                const R y0 = (y + 0.5) / st.lny * 2.03f + 0.02f;
+                const R z0 = (z + 0.5) / st.lnz * 1.64f - 0.82f;
+                const R len = std::min(std::min(slab_exit(x0, dx, -1.05f, 1.05f), slab_exit(y0, dy, 0.02f, 2.05f)), slab_exit(z0, dz, -0.82f, 0.82f));
+                R tau = 0.0;
+                for (int k = 0;; ++k) {
+                    const R s = ds * (k + 0.5);
+                    const R wk = std::min<R>(1.0, 0.5 + (len - s) / ds);
+                    if (wk <= 0.0) break;
+                    const R u = (x0 + dx * s + 1.05f) / 2.10f, v = (y0 + dy * s - 0.02f) / 2.03f, w = (z0 + dz * s + 0.82f) / 1.64f;
+                    tau += wk * sample_grid(ext.data(), st.lnx, st.lny, st.lnz, st.lnx - 1, st.lny - 1, st.lnz - 1, u, v, w);
+                }
+                st.dirT[st.li(x, y, z)] = std::exp(-kDirKappa * ds * tau);
+            }
+}
+static inline R sample_dir_trans(const State& st, R u, R v, R w) {
+    return sample_grid(st.dirT.data(), st.lnx, st.lny, st.lnz, st.lnx - 1, st.lny - 1, st.lnz - 1, u, v, w);
+}
+
 // Build, then two propagations (0.48, 0.92) and (0.34, 0.96) (:3313-3314).
 static void light_pass(State& st, const P& p) {
     const size_t lc = st.lcells();
@@ -1097,6 +1210,7 @@ static void light_pass(State& st, const P& p) {
     light_build(st, p, L);
     light_propagate(st, L, tmp, 0.48f, 0.92f);
     light_propagate(st, tmp, st.light, 0.34f, 0.96f);
+    if (p.dirLight) dir_transmittance(st, p);
 }
 
 // -------------------------------------------------------------- render ----
@@ -1318,6 +1432,11 @@ static V3 room_background(const State& st, const P& p, R uvx, R uvy, V3 rd, R gl
     const R lift = 0.66f + smoothstep(0.006f, 0.13f, luminance3(indirect)) * 0.72f;
     const R warmth = surface == 1 ? 1.72f : (surface == 2 ? 0.88f : 1.16f);
     color = add3(color, mul3(mul3v(indirect, albedo), giScale * grazingFalloff * contactOcc * lift * warmth));
+    if (p.dirLight) {
+        const R ndl = std::max<R>(0.0, -(normal.x * p.dirLightDir[0] + normal.y * p.dirLightDir[1] + normal.z * p.dirLightDir[2]));
+        const R td = sample_dir_trans(st, vuv.x, vuv.y, vuv.z);
+        color = add3(color, mul3(mul3v(albedo, v3(p.dirLightColor[0], p.dirLightColor[1], p.dirLightColor[2])), ndl * td * kDirDiffuse));
+    }
     const R shadowedSoot = (1.0 - volVis) * smoothstep(0.18f, 0.82f, 1.0 - sceneShadow);
     color = lerp3(color, v3(0.006f, 0.007f, 0.008f), shadowedSoot * (surface == 2 ? 0.38f : 0.26f));
     const R wallEdge = surface == 3 ? roomHalf - std::fabs(hit.z) : roomHalf - std::fabs(hit.x);
@@ -1549,6 +1668,10 @@ static V4 render_pixel(const State& st, const P& p, int x, int y) {
             accum = add3(accum, mul3(v3(1.0, 0.24f, 0.035f), trans * coalGlow * stepT));
             accum = add3(accum, mul3(smokeColor, trans * scatterGain * smokeOnly));
             accum = add3(accum, mul3(smokeLight, trans * sceneScatter * (0.18f + warmDamp * 0.42f)));
+            if (p.dirLight) {
+                const R td = sample_dir_trans(st, wu, wv, ww);
+                accum = add3(accum, mul3(v3(p.dirLightColor[0], p.dirLightColor[1], p.dirLightColor[2]), trans * scatterAlpha * td * kDirScatter));
+            }
             flameGlow = std::max(flameGlow, flameDensity * radiantPower * (0.12f + filament * 0.18f + whiteFil * 0.20f));
             smokeOcc = sat(smokeOcc + smokeAlpha * trans);
             trans *= std::exp(-extinction);
@@ -1717,6 +1840,37 @@ static V3 camera_response(V3 radiance, R exposure) {
 }
 
 // tonemapKernel src/fire_cuda.cu:2276-2297 (dither from the pixel hash).
+// drawGizmos src/fire_cuda.cu:1909-1951 over the unpacked 8-bit colour
+// (overlayKernel :2468-2494, after the tonemap :3419-3425).
+static inline R segment_alpha(R px, R py, R ax, R ay, R bx, R by, R thickness) {
+    const R pax = px - ax, pay = py - ay, bax = bx - ax, bay = by - ay;
+    const R h = sat((pax * bax + pay * bay) / std::max<R>(0.000001f, bax * bax + bay * bay));
+    const R dx = pax - bax * h, dy = pay - bay * h;
+    return smoothstep(thickness, thickness * 0.35f, std::sqrt(dx * dx + dy * dy));
+}
+static void draw_gizmos(V3& color, R uvx, R uvy, const P& p) {
+    const Cam& cam = p.cam;
+    const bool burner = p.sceneId == 2 && p.burnerCount > 0;
+    const R sx = burner ? p.burnerX[0] : p.emitterCenterX;
+    const R sy = burner ? p.burnerY[0] : p.emitterHeightNorm * 2.03f + 0.02f;
+    const R sz = burner ? p.burnerZ[0] : p.emitterCenterZ;
+    const V3 c = project_point(cam, v3(sx, sy, sz));
+    if (c.z > 0.0) {
+        const R dist = std::sqrt((uvx - c.x) * (uvx - c.x) + (uvy - c.y) * (uvy - c.y));
+        const R a = smoothstep(0.006f, 0.006f * 0.25f, std::fabs(dist - 0.125f / c.z)) + smoothstep(0.012f, 0.012f * 0.45f, dist);
+        const V3 tint = p.activeGizmo == 1 ? v3(1.0f, 0.34f, 0.055f) : v3(0.45f, 0.18f, 0.055f);
+        color = lerp3(color, tint, sat(a * 0.85f));
+    }
+    const V3 ex = v3(1, 0, 0), ey = v3(0, 1, 0), ez = v3(0, 0, 1);
+    const R bx = 0.90f, by = 0.84f;
+    const V3 axes[3] = {ex, ey, ez};
+    const V3 tints[3] = {v3(1.0f, 0.12f, 0.08f), v3(0.22f, 1.0f, 0.30f), v3(0.20f, 0.48f, 1.0f)};
+    for (int k = 0; k < 3; ++k) {
+        const R tx = bx + dot3(axes[k], cam.right) * 0.070f, ty = by - dot3(axes[k], cam.up) * 0.070f;
+        color = lerp3(color, tints[k], sat(segment_alpha(uvx, uvy, bx, by, tx, ty, 0.005f)));
+    }
+}
+
 static void tonemap(const State& st, const R* hdr, uint32_t* out) {
     const P p = make_p(st, st.renderTime);
 #pragma omp parallel for schedule(static)
@@ -1726,9 +1880,16 @@ static void tonemap(const State& st, const R* hdr, uint32_t* out) {
             V3 c = camera_response(v3(hdr[4 * i], hdr[4 * i + 1], hdr[4 * i + 2]), p.exposure);
             const R dither = (hash2(x, y, p.saltDither) - 0.5f) / 255.0f;
             c = add3(c, v3(dither, dither, dither));
-            const uint32_t r = static_cast<uint32_t>(sat(c.x) * 255.0f);
-            const uint32_t g = static_cast<uint32_t>(sat(c.y) * 255.0f);
-            const uint32_t b = static_cast<uint32_t>(sat(c.z) * 255.0f);
+            uint32_t r = static_cast<uint32_t>(sat(c.x) * 255.0f);
+            uint32_t g = static_cast<uint32_t>(sat(c.y) * 255.0f);
+            uint32_t b = static_cast<uint32_t>(sat(c.z) * 255.0f);
+            if (p.showGizmos && p.debugMode == 0) {
+                V3 g8 = v3(r / 255.0f, g / 255.0f, b / 255.0f);
+                draw_gizmos(g8, (x + 0.5) / st.fw, (y + 0.5) / st.fh, p);
+                r = static_cast<uint32_t>(sat(g8.x) * 255.0f);
+                g = static_cast<uint32_t>(sat(g8.y) * 255.0f);
+                b = static_cast<uint32_t>(sat(g8.z) * 255.0f);
+            }
             out[i] = 0xff000000u | (r << 16) | (g << 8) | b;
         }
     }
@@ -1873,6 +2034,7 @@ void* or_create(int nx, int ny, int nz, int fw, int fh) {
     st->d.assign(st->cells(), 0.0);
     st->light.assign(4 * st->lcells(), 0.0);
     st->shadow.assign(st->lcells(), 0.0);
+    st->dirT.assign(st->lcells(), 0.0);
     return st;
 }
 
@@ -1926,6 +2088,7 @@ static std::vector<R>* field_ptr(State* st, int f) {
         case FS_FIELD_DIVERGENCE: return &st->d;
         case FS_FIELD_LIGHT: return &st->light;
         case FS_FIELD_SHADOW: return &st->shadow;
+        case FS_FIELD_DIR_TRANSMITTANCE: return &st->dirT;
         default: return nullptr;
     }
 }
@@ -2030,6 +2193,11 @@ void or_light_propagate(void* h, double diffusion, double decay) {
     std::vector<R> out(4 * st.lcells());
     light_propagate(st, st.light, out, diffusion, decay);
     st.light.swap(out);
+}
+// Directional-light transmittance of the current light volume (any dir_light).
+void or_dir_transmittance(void* h) {
+    State& st = *static_cast<State*>(h);
+    dir_transmittance(st, make_p(st, st.time));
 }
 // Room colour along ray rd from the scene camera, for pixel uv (render time clock).
 void or_room_background(void* h, double uvx, double uvy, const double* rd, double glow, double* rgb) {

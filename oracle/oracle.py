@@ -19,7 +19,8 @@ sys.path.insert(0, os.path.join(REPO, "native-firesim_amd"))
 
 import firesim  # noqa: E402  (boundary structs: fs_params, fs_emitter, fs_metrics)
 
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB points the tests at another build (the mutation sweep's mutants)
+LIB_PATH = os.environ.get("ORACLE_LIB", os.path.join(HERE, "liboracle.so"))
 
 SALT_NOISE2, SALT_NOISE3, SALT_JITTER, SALT_DITHER, SALT_GRAIN = 1, 2, 3, 4, 5
 
@@ -75,6 +76,7 @@ def load():
         "or_bed_source": (c_double, [c_void_p, c_double, c_double, c_double, c_double]),
         "or_light_build": (None, [c_void_p]),
         "or_light_propagate": (None, [c_void_p, c_double, c_double]),
+        "or_dir_transmittance": (None, [c_void_p]),
         "or_room_background": (None, [c_void_p, c_double, c_double, D, c_double, D]),
         "or_camera_eye": (None, [c_void_p, D]),
     }
@@ -184,7 +186,7 @@ class OracleScene:
         return m.as_dict()
 
     def state(self) -> dict:
-        return {name: self.read(name) for name in firesim.FIELD_NAMES}
+        return {name: self.read(name) for name in firesim.STATE_FIELDS}
 
     def load_state(self, state: dict):
         for name, arr in state.items():

@@ -18,7 +18,7 @@ def test_exports_every_declared_symbol():
     assert len(declared) >= 30
     missing = [name for name in declared if not hasattr(LIB, name)]
     assert not missing, missing
-    assert LIB.fs_abi_version() == 1
+    assert LIB.fs_abi_version() == 2
 
 
 def test_default_params_are_reference_defaults():
@@ -46,8 +46,8 @@ def test_scene_bytes_accounting():
     assert LIB.fs_scene_bytes(ctypes.byref(d), ctypes.byref(b)) == 0
     fp32 = b.value
     cells = 128 ** 3
-    # 20 scalar + 6 face + 6 cell-sized scratch/pressure arrays, plus light/frame/partials
-    assert 32 * 4 * cells < fp32 < 34 * 4 * cells + 16 * 1024 * 1024
+    # 20 scalar + 6 face + 6 cell-sized scratch/pressure + 3 curl-potential edge arrays, plus light/frame/partials
+    assert 35 * 4 * cells < fp32 < 37 * 4 * cells + 16 * 1024 * 1024
     d.storage = firesim.STORAGE_BF16
     assert LIB.fs_scene_bytes(ctypes.byref(d), ctypes.byref(b)) == 0
     assert fp32 - b.value == 6 * 2 * cells  # heat, fuel, soot in both buffers at 2 B
