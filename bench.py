@@ -5,10 +5,11 @@ A step is one pass of the simulation step (velocity advection, advect-react,
 force/boundary/divergence, red-black SOR, projection, light volume) over one
 synthetic scene, through the C ABI (include/firesim.h). The workload is the
 128^3 FP32 scene: one emitter sphere (r=12 voxels at (64,12,64), fuel 1/s,
-1500 K), seeded wind of 0-2 voxels/s in a random xz direction, turbulence
-forcing of amplitude 0.5, SOR 80 iterations at omega 1.7, dt = 1/60. Fields are
-resident in HBM; the timed region is `--steps` steps bracketed by device
-synchronisation, after `--warmup` untimed steps.
+1500 K), seeded wind of 0-2 voxels/s in a random xz direction, divergence-free
+curl-noise forcing of amplitude 0.5, SOR 80 iterations at omega 1.7,
+dt = 1/60, a 200-step run. Fields are resident in HBM; the timed region is
+`--steps` steps bracketed by device synchronisation: the last `--steps` of the
+200-step run (or those after `--warmup` untimed steps, if that is later).
 
 Printed: one JSON line with the contract keys plus
   roofline      the dominant pass (largest GPU time) against HBM peak, its
@@ -69,19 +70,58 @@ def workload(name: str, seed: int):
     return workloads.single_scene(name, seed)
 
 
-def cpu_baseline(name: str, seed: int, budget_s: float):
-    """FP64 oracle on this host's cores: 2 warm-up steps, then timed steps until
-    `budget_s` seconds or 20 steps."""
+def cpu_share() -> int:
+    """CPUs this process may use: its affinity mask, capped by a cgroup CPU quota."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, n)
+
+
+def native_oracle() -> tuple[str | None, str]:
+    """Builds the oracle for this host's CPU (-march=native) into a temporary
+    directory; returns (library path or None, how it was built)."""
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle.so")
+    cmd = ["g++", "-O3", "-march=native", "-fopenmp", "-fPIC", "-std=c++17", "-shared", "-o", out,
+           os.path.join(REPO, "oracle", "oracle.cpp")]
+    try:
+        r = subprocess.run(cmd, capture_output=True, timeout=300)
+    except (OSError, subprocess.TimeoutExpired):
+        return None, "prebuilt x86-64-v3 (native build failed to run)"
+    if r.returncode != 0:
+        return None, "prebuilt x86-64-v3 (native build failed)"
+    return out, "g++ -O3 -march=native -fopenmp"
+
+
+def cpu_baseline(name: str, seed: int, budget_s: float, state=None, start_step: int = 0):
+    """FP64 oracle on this host's CPU share, built for this CPU: from `state`
+    (the GPU's fields at step `start_step`, so both time the same developed
+    fire) or from reset, one untimed step, then timed steps until `budget_s`
+    seconds or 20 steps."""
+    lib, build = native_oracle()
+    if lib:
+        os.environ["ORACLE_LIB"] = lib
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_share()
     oracle.set_threads(threads)
     cfg, params, emitters = workload(name, seed)
     o = oracle.OracleScene(*cfg["grid"], 32, 32)
     o.set_params(params)
     o.set_emitters(emitters)
     o.reset()
-    o.step(2)
+    if state is not None:
+        fields, t, rt = state
+        for k, a in fields.items():
+            o.write(k, a.astype(np.float64))
+        o.set_time(t, rt)
+    o.step(1)
     n, t0 = 0, time.perf_counter()
     while n < 20:
         o.step(1)
@@ -91,9 +131,11 @@ def cpu_baseline(name: str, seed: int, budget_s: float):
     dt = time.perf_counter() - t0
     cells = math.prod(cfg["grid"])
     o.close()
+    origin = f"the GPU state at step {start_step}" if state is not None else "reset"
     return {"value": cells * n / dt, "unit": "voxel-steps/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
-            "host_threads": os.cpu_count(),
-            "sample": f"FP64 oracle, same {cfg['grid'][0]}x{cfg['grid'][1]}x{cfg['grid'][2]} workload from reset: 2 untimed + {n} timed steps, {dt:.1f} s"}
+            "host_logical_cpus": os.cpu_count(), "host_physical_cores": physical_cores(), "build": build,
+            "sample": f"FP64 oracle, same {cfg['grid'][0]}x{cfg['grid'][1]}x{cfg['grid'][2]} workload from {origin}: "
+                      f"1 untimed + {n} timed steps, {dt:.1f} s, {threads} OpenMP threads (this job's CPU share)"}
 
 
 def profile_kernel_name(sor_kernel: str) -> str:
@@ -122,15 +164,44 @@ def read_traffic(kernel_names, grid):
     return None
 
 
-def cpu_model() -> str:
+def _cpuinfo_first() -> dict:
+    """Key/value pairs of the first processor in /proc/cpuinfo."""
+    out = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    out[k.strip()] = v.strip()
     except OSError:
         pass
-    return "unknown"
+    return out
+
+
+def cpu_model() -> str:
+    """Model name with vendor, family, model and stepping numbers and the widest SIMD."""
+    info = _cpuinfo_first()
+    flags = info.get("flags", "").split()
+    simd = "avx512f" if "avx512f" in flags else ("avx2" if "avx2" in flags else "sse")
+    return (f"{info.get('model name', 'unknown')} ({info.get('vendor_id', '?')} family {info.get('cpu family', '?')} "
+            f"model {info.get('model', '?')} stepping {info.get('stepping', '?')}, {simd})")
+
+
+def physical_cores() -> int | None:
+    """Distinct (physical id, core id) pairs in /proc/cpuinfo."""
+    cores, phys = set(), None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("physical id"):
+                    phys = line.split(":", 1)[1].strip()
+                elif line.startswith("core id"):
+                    cores.add((phys, line.split(":", 1)[1].strip()))
+    except OSError:
+        return None
+    return len(cores) or None
 
 
 def calibrate_copy():
@@ -266,7 +337,8 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps before the timed ones; raised so the timed steps end at the configuration's last step")
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--sor-kernel", type=int, default=None, help="enum fs_sor_kernel (default: library default)")
@@ -315,12 +387,18 @@ def main():
     scene.set_params(params)
     scene.set_emitters(emitters)
     scene.reset()
-    if args.warmup:
-        scene.step(args.warmup)
+    # The timed steps are the last --steps of the configuration's run (a young
+    # fire is cheaper: cold cells take the reaction's early-out), or follow
+    # --warmup untimed steps when that is later.
+    start = max(args.warmup, cfg["steps"] - args.steps)
+    if start:
+        scene.step(start)
     scene.synchronize()
     tuned = None
     if args.sor_kernel is None and not args.no_autotune:
         tuned = scene.autotune(reps=2)  # picks the fastest bit-identical SOR kernel for this grid
+    # the same developed state seeds the CPU baseline
+    start_state = ({n: scene.read(n) for n in firesim.STATE_FIELDS}, *scene.get_time()[:2])
 
     # Timed region: the production path (captured step graph replayed per
     # step), HIP events between steps for the latency distribution.
@@ -371,7 +449,7 @@ def main():
     scene.close()
 
     progress("per-pass timing and render done")
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.config, args.seed, args.cpu_budget)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args.config, args.seed, args.cpu_budget, start_state, start)
     progress("cpu baseline done")
     batched = None if args.no_batched else bench_batched(args.seed, args.batched_steps, args.batched_scenes)
     progress("batched done")
@@ -384,7 +462,7 @@ def main():
         "unit": "voxel-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": start,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -396,6 +474,7 @@ def main():
                    "sor_kernel": firesim.SOR_KERNELS[scene_sor_kernel],
                    "sor_autotune_ms": {k: round(v, 4) for k, v in tuned["sor_ms"].items()} if tuned else None,
                    "step_path": "captured HIP graph replayed per step",
+                   "timed_steps": f"{start + 1}-{start + args.steps} of the {cfg['steps']}-step run ({start} untimed steps first)",
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

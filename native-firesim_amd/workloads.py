@@ -15,13 +15,19 @@ import numpy as np
 
 import firesim
 
+# steps: the length of the run the configuration names; extra: fs_params extensions
 SINGLE = {
-    "c1": dict(grid=(64, 64, 64), sor=40, turbulence=0.0, wind=False, emitters="single", render=(256, 256, 104),
-               desc="64^3 FP32, emitter r=6 at (32,6,32), SOR 40 x omega 1.7"),
-    "c2": dict(grid=(128, 128, 128), sor=80, turbulence=0.5, wind=True, emitters="single", render=(512, 512, 128),
-               desc="128^3 FP32, emitter r=12 at (64,12,64), seeded wind 0-2 voxel/s, turbulence 0.5, SOR 80 x omega 1.7"),
-    "c3": dict(grid=(256, 384, 256), sor=120, turbulence=0.5, wind=True, emitters="eight", render=(1280, 720, 104),
-               desc="256x384x256 FP32, 8 seeded floor emitters, SOR 120 x omega 1.7"),
+    "c1": dict(grid=(64, 64, 64), sor=40, turbulence=0.0, wind=False, emitters="single", render=(256, 256, 104), steps=100,
+               extra=dict(closed_top=1, noise_forcing=1, curl_noise=0.0, dir_light=1, dir_light_x=0.35, dir_light_y=-1.0,
+                          dir_light_z=0.25, dir_light_r=1.6, dir_light_g=1.5, dir_light_b=1.35),
+               desc="64^3 FP32, emitter r=6 at (32,6,32), closed walls, no noise forcing, SOR 40 x omega 1.7, "
+                    "render with a directional light"),
+    "c2": dict(grid=(128, 128, 128), sor=80, turbulence=0.5, wind=True, emitters="single", render=(512, 512, 128), steps=200,
+               extra=dict(noise_forcing=1, curl_noise=0.5),
+               desc="128^3 FP32, emitter r=12 at (64,12,64), seeded wind 0-2 voxel/s, divergence-free curl-noise forcing "
+                    "of amplitude 0.5, SOR 80 x omega 1.7"),
+    "c3": dict(grid=(256, 384, 256), sor=120, turbulence=0.5, wind=True, emitters="eight", render=(1280, 720, 104), steps=100,
+               extra={}, desc="256x384x256 FP32, 8 seeded floor emitters, SOR 120 x omega 1.7"),
 }
 
 BATCH_SHAPES = [(48, 48, 48), (64, 64, 64), (96, 96, 96), (128, 128, 128), (64, 128, 64)]
@@ -42,10 +48,10 @@ def floor_emitters(rng, nx, nz, count, r_lo, r_hi):
     return out
 
 
-def params_for(seed, sor, turbulence, wind=(0.0, 0.0), raymarch=64):
+def params_for(seed, sor, turbulence, wind=(0.0, 0.0), raymarch=64, **extra):
     return firesim.default_params(scene_id=firesim.SCENE_EMITTERS_ONLY, turbulence=turbulence, wind_x=wind[0], wind_z=wind[1],
                                   sor_omega=1.7, sor_iterations=sor, raymarch_steps=raymarch, noise_seed=seed,
-                                  camera_distance=3.0, cinematic_mode=1)
+                                  camera_distance=3.0, cinematic_mode=1, **extra)
 
 
 def single_scene(name: str, seed: int):
@@ -59,7 +65,7 @@ def single_scene(name: str, seed: int):
         emitters = [(nx / 2, r, nz / 2, r, 1.0, 1500.0)]
     else:
         emitters = floor_emitters(rng, nx, nz, 8, 4.0, 10.0)
-    return cfg, params_for(seed, cfg["sor"], cfg["turbulence"], wind, cfg["render"][2]), emitters
+    return cfg, params_for(seed, cfg["sor"], cfg["turbulence"], wind, cfg["render"][2], **cfg["extra"]), emitters
 
 
 def batch_scenes(seed: int, count: int = 48):

@@ -102,3 +102,16 @@ def test_copy_bandwidth_rejects_bad_sizes_before_touching_a_device():
         assert LIB.fs_copy_bandwidth(0, nbytes, reps, ctypes.byref(out)) == 1, (nbytes, reps)
         assert out.value == 0.0
     assert LIB.fs_copy_bandwidth(0, 1 << 20, 1, None) == 1
+
+
+def test_no_packed_fp32_in_the_device_code():
+    """The gfx950 code objects hold no v_pk_{add,mul,fma}_f32 (the build switches
+    the packed-fp32-ops feature off; the checker disassembles each object)."""
+    import glob
+    import subprocess
+    pkg = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "native-firesim_amd")
+    objs = sorted(glob.glob(os.path.join(pkg, "_build", "*.hip.o")))
+    if not objs:
+        pytest.skip("objects not built")
+    r = subprocess.run([os.path.join(pkg, "check_packed_fp32.sh"), *objs], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

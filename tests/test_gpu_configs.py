@@ -1,0 +1,147 @@
+"""GPU parity at the workload configurations' own sizes, teacher-forced from the
+GPU's developed state (the FP64 oracle would need hours to reach it):
+
+  * 64^3, closed walls, no noise forcing, SOR 40: 100 steps, then one step of
+    each side from the GPU state; the 256x256 render with the directional light;
+  * 128^3 with curl noise, seeded wind, SOR 80 on the autotuned SOR tile and the
+    captured step graph: 199 steps, then the 200th step on each side from the
+    GPU state; the 512x512 frame with 128 samples per ray;
+  * 256x384x256 with 8 seeded emitters, SOR 120: 99 steps, then one step of
+    each side; the 1280x720 frame;
+  * a served BF16 request stream with grids up to 192^3: the largest request's
+    scene after its last served step, one more step on each side.
+
+Bounds as in the other parity tests: relative L2 <= 1e-5 per field for FP32
+steps, 1e-3 for HDR frames; BF16 storage with the bounds of the batched
+precision test (heat, fuel, soot carry the 8-bit mantissa rounding).
+"""
+import numpy as np
+import pytest
+
+import compare
+import firesim
+import oracle
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+BF16_BOUND = {"heat": 4e-3, "fuel": 4e-3, "soot": 4e-3, "u": 2e-3, "v": 2e-3, "w": 2e-3, "pressure": 2e-3, "divergence": 1e-2,
+              "light": 5e-2}
+
+
+def _gpu_then_oracle(name, seed, before, autotune):
+    """GPU scene of workload `name` stepped `before` times (graph path), and an
+    oracle loaded with its state."""
+    cfg, params, emitters = workloads.single_scene(name, seed)
+    fw, fh, _ = cfg["render"]
+    s = firesim.Scene(*cfg["grid"], fw, fh)
+    s.set_params(params)
+    s.set_emitters(emitters)
+    s.reset()
+    if autotune:
+        s.step(4)
+        s.autotune(reps=1)
+    s.step(before - (4 if autotune else 0))
+    s.synchronize()
+    o = oracle.OracleScene(*cfg["grid"], fw, fh)
+    o.set_params(params)
+    o.set_emitters(emitters)
+    o.reset()
+    for n in firesim.STATE_FIELDS:
+        o.write(n, s.read(n).astype(np.float64))
+    t, rt, _ = s.get_time()
+    o.set_time(t, rt)
+    return cfg, params, s, o
+
+
+def _step_and_compare(s, o, label, extra=()):
+    s.step(1)
+    o.step(1)
+    errs = compare.compare_fields(s, o, list(compare.STEP_FIELDS) + list(extra))
+    print(label, {k: float(f"{v:.3g}") for k, v in errs.items()}, flush=True)
+    bad = {k: v for k, v in errs.items() if not v <= TOL}
+    assert not bad, f"{label}: {bad}"
+
+
+def _render_compare(s, o, label):
+    # identical inputs: the GPU's state after the compared step
+    for n in list(firesim.STATE_FIELDS) + ["dir_transmittance"]:
+        o.write(n, s.read(n).astype(np.float64))
+    hs, ho = s.render_hdr(), o.render_hdr()
+    err = compare.rel_l2(hs[..., :3], ho[..., :3])
+    print(label, "hdr rel l2", err, "mean", float(ho[..., :3].mean()), flush=True)
+    assert ho[..., :3].max() > 0.05
+    assert err <= 1e-3
+
+
+def test_config_64_closed_walls_directional_light(hip_lib):
+    cfg, params, s, o = _gpu_then_oracle("c1", 3, 99, autotune=False)
+    assert params.closed_top == 1 and params.dir_light == 1 and params.noise_forcing == 1 and params.curl_noise == 0.0
+    _step_and_compare(s, o, "64^3 step 100", extra=["dir_transmittance"])
+    assert np.all(s.read("v")[:, -1, :] == 0)
+    _render_compare(s, o, "64^3 256x256")
+    s.close()
+    o.close()
+
+
+def test_config_128_curl_noise_graph_autotuned(hip_lib):
+    cfg, params, s, o = _gpu_then_oracle("c2", 2, 199, autotune=True)
+    assert params.noise_forcing == 1 and params.curl_noise == pytest.approx(0.5) and params.sor_iterations == 80
+    print("SOR kernel", firesim.SOR_KERNELS[s.sor_kernel()], flush=True)
+    _step_and_compare(s, o, "128^3 step 200")
+    m = s.metrics()
+    assert m["invalid_cells"] == 0 and m["max_heat"] > 1.0
+    _render_compare(s, o, "128^3 512x512x128")
+    s.close()
+    o.close()
+
+
+def test_config_256x384x256_eight_emitters(hip_lib):
+    cfg, params, s, o = _gpu_then_oracle("c3", 2, 99, autotune=True)
+    assert params.sor_iterations == 120
+    _step_and_compare(s, o, "256x384x256 step 100")
+    _render_compare(s, o, "256x384x256 1280x720")
+    s.close()
+    o.close()
+
+
+def test_config_served_bf16_up_to_192(hip_lib):
+    """A served stream (BF16 storage, edges 32..192, a frame every 10 steps);
+    the 192^3 request keeps its scene, which then steps once more beside the
+    oracle loaded with its state."""
+    reqs = workloads.served_requests(seed=7, count=6, rate_per_s=200.0)
+    reqs[0]["grid"] = (192, 192, 192)
+    reqs[0]["emitters"] = [(96.0, 18.0, 96.0, 18.0, 1.0, 1500.0)]
+    reqs[0]["steps"] = 40
+    for r in reqs[1:]:
+        r["steps"] = min(r["steps"], 30)
+    with firesim.Server(hbm_budget_bytes=200 << 30, max_active=16) as sv:
+        ids = []
+        for i, r in enumerate(reqs):
+            ids.append(sv.submit(firesim.make_request(*r["grid"], r["params"], r["emitters"], steps=r["steps"],
+                                                      render_every=r["render_every"], frame=r["frame"], storage=firesim.STORAGE_BF16,
+                                                      keep_scene=(i == 0), arrival_s=-1.0)))
+        sv.run()
+        assert all(sv.status(i)["state"] == firesim.REQ_DONE for i in ids)
+        s = sv.take_scene(ids[0])
+    try:
+        assert (s.nx, s.ny, s.nz) == (192, 192, 192)
+        o = oracle.OracleScene(192, 192, 192, 32, 32)
+        o.set_params(reqs[0]["params"])
+        o.set_emitters(reqs[0]["emitters"])
+        o.reset()
+        for n in firesim.STATE_FIELDS:
+            o.write(n, s.read(n).astype(np.float64))
+        t, rt, fi = s.get_time()
+        assert fi == 40
+        o.set_time(t, rt)
+        s.step(1)
+        o.step(1)
+        errs = compare.compare_fields(s, o, compare.STEP_FIELDS)
+        print("served 192^3 bf16 step 41", {k: float(f"{v:.3g}") for k, v in errs.items()}, flush=True)
+        bad = {k: v for k, v in errs.items() if not v <= BF16_BOUND.get(k, TOL)}
+        assert not bad, bad
+        o.close()
+    finally:
+        s.close()
