@@ -58,7 +58,7 @@ PASS_KERNELS = {
     "advect_velocity": ["k_advect_velocity"],
     "advect_react": ["k_advect_react", "k_finalize_release"],
     "force": ["k_force_contrib", "k_force_apply"],
-    "pressure": ["k_sor_tb", "k_sor_half"],
+    "pressure": ["k_sor_tb", "k_sor_col", "k_sor_half"],
     "project": ["k_project"],
     "light": ["k_light_build", "k_light_propagate"],
 }
@@ -140,11 +140,13 @@ def cpu_baseline(name: str, seed: int, budget_s: float, state=None, start_step: 
 
 def profile_kernel_name(sor_kernel: str) -> str:
     """rocprofv3's short name of the SOR kernel a scene runs: tb_t24_z16_k4 ->
-    k_sor_tb<24,4,16> (tile edge, half-sweeps per launch, tile depth)."""
+    k_sor_tb<24,4,16>, col_t24_z12_k4 -> k_sor_col<24,4,12> (tile edge,
+    half-sweeps per launch, tile depth)."""
     if sor_kernel == "half_sweep":
         return "k_sor_half"
-    parts = dict((tok[0], int(tok[1:])) for tok in sor_kernel.split("_")[1:])
-    return f"k_sor_tb<{parts['t']},{parts['k']},{parts.get('z', parts['t'])}>"
+    form, *rest = sor_kernel.split("_")
+    parts = dict((tok[0], int(tok[1:])) for tok in rest)
+    return f"k_sor_{form}<{parts['t']},{parts['k']},{parts.get('z', parts['t'])}>"
 
 
 def read_traffic(kernel_names, grid):
@@ -283,17 +285,13 @@ def bench_batched(seed: int, steps: int, count: int):
     return result
 
 
-def bench_served(seed: int, count: int, rate: float, budget_gb: float, storage: int, use_graph: bool = True):
-    """Served stream: Poisson requests through fs_server (continuous batching,
-    HBM-budget admission, a frame every 10 steps), then the same requests
-    serially through the plain FP32 path."""
-    reqs = workloads.served_requests(seed, count, rate)
-    total_voxel_steps = sum(math.prod(r["grid"]) * r["steps"] for r in reqs)
+def _serve(reqs, budget_gb: float, storage: int, use_graph: bool, max_round_voxels: int = 0) -> dict:
+    """One pass of the request stream through fs_server; its statistics."""
     # A fixed SOR tile, 24x24x12, the fastest batched tile on this request mix
     # (mostly large scenes): timing every kernel on the resident mix inside the
     # stream would stall its rounds.
     with firesim.Server(hbm_budget_bytes=int(budget_gb * (1 << 30)), max_active=256, sor_kernel=firesim.SOR_KERNELS.index("tb_t24_z12_k4"),
-                        use_graph=use_graph) as sv:
+                        use_graph=use_graph, max_round_voxels=max_round_voxels) as sv:
         for r in reqs:
             sv.submit(firesim.make_request(*r["grid"], r["params"], r["emitters"], steps=r["steps"], render_every=r["render_every"],
                                            frame=r["frame"], storage=storage, arrival_s=r["arrival"]))
@@ -301,16 +299,34 @@ def bench_served(seed: int, count: int, rate: float, budget_gb: float, storage: 
         sv.run()
         wall = time.perf_counter() - t0
         st = sv.stats()
+    return {"voxel_steps": st["voxel_steps"], "busy_s": round(st["busy_s"], 3), "wall_s": round(wall, 3),
+            "voxel_steps_per_s": st["voxel_steps"] / st["busy_s"] if st["busy_s"] > 0 else None,
+            "step_ms": {"p50": round(st["step_ms_p50"], 3), "p99": round(st["step_ms_p99"], 3), "mean": round(st["step_ms_mean"], 3),
+                        "max": round(st["step_ms_max"], 3)},
+            "queue_ms": {"p50": round(st["queue_ms_p50"], 3), "p99": round(st["queue_ms_p99"], 3)},
+            "max_active": st["max_active_seen"], "rounds": st["rounds"], "frames": st["frames"],
+            "peak_resident_gb": round(st["peak_resident_bytes"] / (1 << 30), 2), "sor_kernel": firesim.SOR_KERNELS[st["sor_kernel"]],
+            "failed": st["requests_failed"]}
+
+
+def bench_served(seed: int, count: int, rate: float, budget_gb: float, storage: int, use_graph: bool = True, target_ms: float = 10.0):
+    """Served stream: Poisson requests through fs_server (continuous batching,
+    HBM-budget admission, a frame every 10 steps), once without a round cap
+    and once with the cap that holds a round to `target_ms` at the uncapped
+    rate (the p99 step latency is reported against that target), then the same
+    requests serially through the plain FP32 path."""
+    reqs = workloads.served_requests(seed, count, rate)
+    total_voxel_steps = sum(math.prod(r["grid"]) * r["steps"] for r in reqs)
     served = {"requests": count, "arrival_rate_per_s": rate, "hbm_budget_gb": budget_gb,
-              "storage": "bf16" if storage == firesim.STORAGE_BF16 else "fp32",
-              "voxel_steps": st["voxel_steps"], "busy_s": round(st["busy_s"], 3), "wall_s": round(wall, 3),
-              "voxel_steps_per_s": st["voxel_steps"] / st["busy_s"] if st["busy_s"] > 0 else None,
-              "step_ms": {"p50": round(st["step_ms_p50"], 3), "p99": round(st["step_ms_p99"], 3), "mean": round(st["step_ms_mean"], 3),
-                          "max": round(st["step_ms_max"], 3)},
-              "queue_ms": {"p50": round(st["queue_ms_p50"], 3), "p99": round(st["queue_ms_p99"], 3)},
-              "max_active": st["max_active_seen"], "rounds": st["rounds"], "frames": st["frames"],
-              "peak_resident_gb": round(st["peak_resident_bytes"] / (1 << 30), 2), "sor_kernel": firesim.SOR_KERNELS[st["sor_kernel"]],
-              "failed": st["requests_failed"]}
+              "storage": "bf16" if storage == firesim.STORAGE_BF16 else "fp32"}
+    served.update(_serve(reqs, budget_gb, storage, use_graph))
+    st = {"voxel_steps": served["voxel_steps"], "requests_failed": served["failed"]}
+    # a quarter of the target left for the frames rendered between steps and graph rebuilds
+    cap = int(served["voxel_steps_per_s"] * target_ms * 1e-3 * 0.75)
+    capped = _serve(reqs, budget_gb, storage, use_graph, max_round_voxels=cap)
+    assert capped["voxel_steps"] == total_voxel_steps and capped["failed"] == 0
+    served["latency_capped"] = dict(capped, target_step_ms=target_ms, max_round_voxels=cap,
+                                    p99_within_target=capped["step_ms"]["p99"] <= target_ms)
     assert st["voxel_steps"] == total_voxel_steps and st["requests_failed"] == 0
     # serially, plain FP32 path, same requests back to back (arrivals ignored: the GPU never waits)
     lat = []
@@ -330,6 +346,7 @@ def bench_served(seed: int, count: int, rate: float, budget_gb: float, storage: 
     serial_wall = time.perf_counter() - t0
     served["plain_fp32_serial"] = {"voxel_steps_per_s": total_voxel_steps / serial_wall, "wall_s": round(serial_wall, 3), "step_ms": pcts(lat)}
     served["speedup_vs_plain_fp32"] = served["voxel_steps_per_s"] / served["plain_fp32_serial"]["voxel_steps_per_s"]
+    served["latency_capped"]["speedup_vs_plain_fp32"] = capped["voxel_steps_per_s"] / served["plain_fp32_serial"]["voxel_steps_per_s"]
     return served
 
 
@@ -353,6 +370,7 @@ def main():
     ap.add_argument("--served-budget-gb", type=float, default=200.0)
     ap.add_argument("--served-fp32", action="store_true", help="serve FP32 fields instead of BF16 heat/fuel/soot")
     ap.add_argument("--served-no-graph", action="store_true", help="direct launches instead of captured batch step graphs")
+    ap.add_argument("--served-target-ms", type=float, default=10.0, help="step latency target of the capped served pass")
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-scene uneven batch")
     ap.add_argument("--batched-scenes", type=int, default=48)
     ap.add_argument("--batched-steps", type=int, default=10)
@@ -372,7 +390,7 @@ def main():
     if args.only == "served":
         storage = firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16
         print(json.dumps({"served": bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb, storage,
-                                                 not args.served_no_graph)}))
+                                                 not args.served_no_graph, args.served_target_ms)}))
         return
     if args.only == "single":
         args.no_batched = args.no_served = True
@@ -455,7 +473,7 @@ def main():
     progress("batched done")
     served = None if args.no_served else bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb,
                                                       firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16,
-                                                      not args.served_no_graph)
+                                                      not args.served_no_graph, args.served_target_ms)
     out = {
         "metric": "voxel-steps/s at 128³ + served p50/p99 step latency vs plain FP32 path",
         "value": value,

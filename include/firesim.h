@@ -415,6 +415,9 @@ typedef struct fs_server_desc {
     uint64_t hbm_budget_bytes;  /* device memory for resident requests (fs_scene_bytes each) */
     int32_t sor_kernel;         /* enum fs_sor_kernel; -1: autotune on the first rounds' members */
     int32_t use_graph;          /* replay captured batch step graphs */
+    uint64_t max_round_voxels;  /* 0: no cap; else a request is admitted only while the admitted
+                                 * requests' cells stay within it (a round's time, hence every
+                                 * step's latency, scales with them); a larger request runs alone */
 } fs_server_desc;
 
 typedef struct fs_request {
@@ -449,7 +452,8 @@ typedef struct fs_server_stats {
     int32_t sor_kernel;         /* SOR kernel in use */
     uint64_t peak_resident_bytes;
     double busy_s;              /* first arrival to last completion */
-    double step_ms_p50, step_ms_p99, step_ms_mean, step_ms_max;
+    double step_ms_p50, step_ms_p99, step_ms_mean, step_ms_max;  /* a step's latency: from the request's
+                                 * previous step completion (its admission, for the first) to this one */
     double queue_ms_p50, queue_ms_p99;   /* arrival to admission */
 } fs_server_stats;
 

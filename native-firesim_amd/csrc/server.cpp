@@ -193,6 +193,11 @@ bool admit(fs_server* sv, Request& q, double t) {
     d.device = sv->desc.device;
     if (static_cast<int>(sv->active.size() + sv->draining.size()) >= sv->desc.max_active) return false;
     if (sv->usedBytes + q.sceneBytes > sv->desc.hbm_budget_bytes) return false;
+    if (sv->desc.max_round_voxels > 0 && !sv->active.empty()) {
+        uint64_t cells = q.cells;
+        for (uint64_t id : sv->active) cells += sv->reqs[id].cells;
+        if (cells > sv->desc.max_round_voxels) return false;
+    }
     // Nothing here waits for the device: the scene is created without its
     // initial reset, joins the batch, and is reset on the batch stream, in
     // order before its first step.
@@ -222,7 +227,7 @@ bool admit(fs_server* sv, Request& q, double t) {
     }
     q.state = FS_REQ_RUNNING;
     q.admitted = t;
-    q.readyAt = std::max(q.arrival, 0.0);
+    q.readyAt = t;  // step latency counts from admission; the wait before it is the queue time
     sv->usedBytes += q.sceneBytes;
     sv->peakBytes = std::max(sv->peakBytes, sv->usedBytes);
     sv->queueMs.push_back((t - q.arrival) * 1e3);

@@ -49,14 +49,37 @@ __device__ __forceinline__ double block_sum_256(double v) {
 }
 
 // ------------------------------------------------------------ K5-K7 ----
-// Velocity (u, v, w) at a position from its per-axis sample setups: F is an
-// axis of the face-aligned array of that component (n + 1 samples), C of the
-// cell-aligned ones (n samples). Same values as sample_vel.
-__device__ __forceinline__ F3 vel_from_axes(const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                                            const DevParams& p, const Axis& xF, const Axis& xC, const Axis& yF, const Axis& yC,
-                                            const Axis& zF, const Axis& zC) {
-    return f3(tri_plain(U, tri_of(xF, yC, zC, p.nx + 1, p.ny)), tri_plain(V, tri_of(xC, yF, zC, p.nx, p.ny + 1)),
-              tri_plain(W, tri_of(xC, yC, zF, p.nx, p.ny)));
+// Sample coordinates in index units, I + r: I an integer cell index and r a
+// small FP32 remainder (the position's offset within the grid spacing plus the
+// backtrace displacement in cells). The reference forms a normalized
+// coordinate u in [0, 1] and scales it by s (n on a face-aligned axis, n - 1 on
+// a cell-aligned one): at x / n that is x on a face axis and x - x/n on a cell
+// axis, at (x + 0.5) / n it is x + 0.5 and x + 0.5 - (x + 0.5)/n. Splitting off
+// the integer keeps the remainder's rounding at FP32 ulp of a number below a
+// few cells (~1e-7 of a cell), where a normalized FP32 coordinate scaled by up
+// to 1024 cells would carry ~6e-5 of a cell; so the backtrace needs no FP64.
+// Clamped to [0, s] as the reference's saturate of u.
+__device__ __forceinline__ Axis axis_at(int d, int s, int I, float r) {
+    const float fl = floorf(r);
+    int i0 = I + static_cast<int>(fl);
+    float t = r - fl;
+    if (i0 < 0) {
+        i0 = 0;
+        t = 0.0f;
+    } else if (i0 >= s) {
+        i0 = s;
+        t = 0.0f;
+    }
+    return Axis{i0, min(i0 + 1, d - 1), t};
+}
+
+// Bilinear sample of a plane-aligned position: the component's own axis lies
+// on a grid node (t = 0 there), so only four corners are read; lerp(a, b, 0)
+// is a, so the value equals the trilinear one.
+__device__ __forceinline__ float bilerp(const float* __restrict__ f, Idx i00, Idx o1, Idx o2, float t1, float t2) {
+    const float a = lerpf(ld_at(f, i00), ld_at(f, i00 + o1), t1);
+    const float b = lerpf(ld_at(f, i00 + o2), ld_at(f, i00 + o2 + o1), t1);
+    return lerpf(a, b, t2);
 }
 
 // Thread (x, y, z) of a face grid of (nx, ny+1, nz+1) writes the u, v and w
@@ -67,48 +90,67 @@ __device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float
                                                    const DevParams& p, int x, int y, int z) {
     if (x >= p.nx || y > p.ny || z > p.nz) return;
     if (x == p.nx - 1 && y < p.ny && z < p.nz) st_at(uo, uidx(p, p.nx, y, z), 0.0f);
-    // backtrace positions in double (see tri_setup); sampled values and outputs in float
-    const double dnx = p.nx, dny = p.ny, dnz = p.nz;
-    const double k = static_cast<double>(p.dt) * static_cast<double>(kAdvectionScale);
-    // The three face positions share their coordinates per axis: x / nx on
-    // the u face, (x + 0.5) / nx on the v and w faces, and so on. Each
-    // coordinate's two axis setups (face- and cell-aligned arrays) are formed
-    // once: 12 axis setups for the nine velocity samples instead of 27.
-    const double ax = x / dnx, cx = (x + 0.5) / dnx;
-    const double ay = y / dny, cy = (y + 0.5) / dny;
-    const double az = z / dnz, cz = (z + 0.5) / dnz;
-    const double snx = static_cast<double>(p.nx), sny = static_cast<double>(p.ny), snz = static_cast<double>(p.nz);
-    const double snx1 = static_cast<double>(p.nx - 1), sny1 = static_cast<double>(p.ny - 1), snz1 = static_cast<double>(p.nz - 1);
-    const Axis axF = axis_setup<double>(p.nx + 1, snx, ax), axC = axis_setup<double>(p.nx, snx1, ax);
-    const Axis cxF = axis_setup<double>(p.nx + 1, snx, cx), cxC = axis_setup<double>(p.nx, snx1, cx);
-    const Axis ayF = axis_setup<double>(p.ny + 1, sny, ay), ayC = axis_setup<double>(p.ny, sny1, ay);
-    const Axis cyF = axis_setup<double>(p.ny + 1, sny, cy), cyC = axis_setup<double>(p.ny, sny1, cy);
-    const Axis azF = axis_setup<double>(p.nz + 1, snz, az), azC = axis_setup<double>(p.nz, snz1, az);
-    const Axis czF = axis_setup<double>(p.nz + 1, snz, cz), czC = axis_setup<double>(p.nz, snz1, cz);
-    if (y < p.ny && z < p.nz) {
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const float k = p.dt * kAdvectionScale;
+    // remainders of the face (a) and centre (c) coordinates on the
+    // cell-aligned axes; on face-aligned axes they are 0 and 0.5
+    const float axr = -static_cast<float>(x) / static_cast<float>(nx), cxr = 0.5f - (static_cast<float>(x) + 0.5f) / static_cast<float>(nx);
+    const float ayr = -static_cast<float>(y) / static_cast<float>(ny), cyr = 0.5f - (static_cast<float>(y) + 0.5f) / static_cast<float>(ny);
+    const float azr = -static_cast<float>(z) / static_cast<float>(nz), czr = 0.5f - (static_cast<float>(z) + 0.5f) / static_cast<float>(nz);
+    const Idx rowU = static_cast<Idx>(nx + 1), planeU = rowU * static_cast<Idx>(ny);
+    const Idx rowV = static_cast<Idx>(nx), planeV = rowV * static_cast<Idx>(ny + 1);
+    const Idx rowW = static_cast<Idx>(nx), planeW = rowW * static_cast<Idx>(ny);
+    // Axis setups shared by the nine velocity samples: per coordinate, the
+    // face-aligned (F) and cell-aligned (C) array axes.
+    const Axis axC = axis_at(nx, nx - 1, x, axr), cxF = axis_at(nx + 1, nx, x, 0.5f), cxC = axis_at(nx, nx - 1, x, cxr);
+    const Axis ayC = axis_at(ny, ny - 1, y, ayr), cyF = axis_at(ny + 1, ny, y, 0.5f), cyC = axis_at(ny, ny - 1, y, cyr);
+    const Axis azC = axis_at(nz, nz - 1, z, azr), czF = axis_at(nz + 1, nz, z, 0.5f), czC = axis_at(nz, nz - 1, z, czr);
+    if (y < ny && z < nz) {
         float out = 0.0f;
-        if (x != 0 && x != p.nx) {
-            const F3 vel = vel_from_axes(U, V, W, p, axF, axC, cyF, cyC, czF, czC);
-            out = sample_u<double>(U, p, ax - vel.x * k, cy - vel.y * k, cz - vel.z * k) * 0.995f;
+        if (x != 0 && x != nx) {
+            // velocity at the u face (x, y + 1/2, z + 1/2): u on its x node
+            const float vu = bilerp(U, (static_cast<Idx>(czC.i0) * static_cast<Idx>(ny) + cyC.i0) * rowU + x,
+                                    cyC.i1 != cyC.i0 ? rowU : 0u, czC.i1 != czC.i0 ? planeU : 0u, cyC.t, czC.t);
+            const float vv = tri_plain(V, tri_of(axC, cyF, czC, nx, ny + 1));
+            const float vw = tri_plain(W, tri_of(axC, cyC, czF, nx, ny));
+            const Tri tb = tri_of(axis_at(nx + 1, nx, x, -vu * k * static_cast<float>(nx)),
+                                  axis_at(ny, ny - 1, y, cyr - vv * k * static_cast<float>(ny - 1)),
+                                  axis_at(nz, nz - 1, z, czr - vw * k * static_cast<float>(nz - 1)), nx + 1, ny);
+            out = tri_plain(U, tb) * 0.995f;
         }
         st_at(uo, uidx(p, x, y, z), out);
     }
-    if (x < p.nx && z < p.nz) {
+    if (x < nx && z < nz) {
         float out = 0.0f;
-        if (y != 0 && !(p.closedTop && y == p.ny)) {
-            const F3 vel = vel_from_axes(U, V, W, p, cxF, cxC, ayF, ayC, czF, czC);
-            out = sample_v<double>(V, p, cx - vel.x * k, ay - vel.y * k, cz - vel.z * k) * 0.996f;
+        if (y != 0 && !(p.closedTop && y == ny)) {
+            // velocity at the v face (x + 1/2, y, z + 1/2): v on its y node
+            const float vu = tri_plain(U, tri_of(cxF, ayC, czC, nx + 1, ny));
+            const float vv = bilerp(V, (static_cast<Idx>(czC.i0) * static_cast<Idx>(ny + 1) + y) * rowV + cxC.i0, static_cast<Idx>(cxC.i1 - cxC.i0),
+                                    czC.i1 != czC.i0 ? planeV : 0u, cxC.t, czC.t);
+            const float vw = tri_plain(W, tri_of(cxC, ayC, czF, nx, ny));
+            const Tri tb = tri_of(axis_at(nx, nx - 1, x, cxr - vu * k * static_cast<float>(nx - 1)),
+                                  axis_at(ny + 1, ny, y, -vv * k * static_cast<float>(ny)),
+                                  axis_at(nz, nz - 1, z, czr - vw * k * static_cast<float>(nz - 1)), nx, ny + 1);
+            out = tri_plain(V, tb) * 0.996f;
         }
         st_at(vo, vidx(p, x, y, z), out);
     }
-    if (x < p.nx && y < p.ny) {
+    if (x < nx && y < ny) {
         float out = 0.0f;
-        if (z != 0 && z != p.nz) {
-            const F3 vel = vel_from_axes(U, V, W, p, cxF, cxC, cyF, cyC, azF, azC);
-            out = sample_w<double>(W, p, cx - vel.x * k, cy - vel.y * k, az - vel.z * k) * 0.995f;
+        if (z != 0 && z != nz) {
+            // velocity at the w face (x + 1/2, y + 1/2, z): w on its z node
+            const float vu = tri_plain(U, tri_of(cxF, cyC, azC, nx + 1, ny));
+            const float vv = tri_plain(V, tri_of(cxC, cyF, azC, nx, ny + 1));
+            const float vw = bilerp(W, (static_cast<Idx>(z) * static_cast<Idx>(ny) + cyC.i0) * rowW + cxC.i0, static_cast<Idx>(cxC.i1 - cxC.i0),
+                                    cyC.i1 != cyC.i0 ? rowW : 0u, cxC.t, cyC.t);
+            const Tri tb = tri_of(axis_at(nx, nx - 1, x, cxr - vu * k * static_cast<float>(nx - 1)),
+                                  axis_at(ny, ny - 1, y, cyr - vv * k * static_cast<float>(ny - 1)),
+                                  axis_at(nz + 1, nz, z, -vw * k * static_cast<float>(nz)), nx, ny);
+            out = tri_plain(W, tb) * 0.995f;
         }
         st_at(wo, widx(p, x, y, z), out);
     }
+    (void)planeW;
 }
 
 __device__ __forceinline__ int3 cell_of(uint3 blk) {

@@ -109,7 +109,7 @@ class Metrics(Structure):
 
 class ServerDesc(Structure):
     _fields_ = [("device", c_int32), ("max_active", c_int32), ("hbm_budget_bytes", c_uint64),
-                ("sor_kernel", c_int32), ("use_graph", c_int32)]
+                ("sor_kernel", c_int32), ("use_graph", c_int32), ("max_round_voxels", c_uint64)]
 
 
 class Request(Structure):
@@ -618,9 +618,9 @@ class Server:
     """Continuous batching of simulation requests (fs_server)."""
 
     def __init__(self, hbm_budget_bytes: int, max_active: int = 256, sor_kernel: int = -1, use_graph: bool = True,
-                 device: int = 0):
+                 device: int = 0, max_round_voxels: int = 0):
         self.lib = load_library()
-        d = ServerDesc(device, max_active, hbm_budget_bytes, sor_kernel, int(use_graph))
+        d = ServerDesc(device, max_active, hbm_budget_bytes, sor_kernel, int(use_graph), max_round_voxels)
         h = c_void_p()
         st = self.lib.fs_server_create(byref(d), byref(h))
         if st != FS_OK:

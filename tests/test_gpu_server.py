@@ -115,3 +115,25 @@ def test_latency_bookkeeping(hip_lib):
     assert 0 < stats["step_ms_p50"] <= stats["step_ms_p99"] <= stats["step_ms_max"] + 1e-9
     assert stats["queue_ms_p50"] >= 0 and stats["busy_s"] > 0
     assert stats["frames"] == 0
+
+
+def test_round_voxel_cap_bounds_the_round(hip_lib):
+    """With max_round_voxels, no round steps more admitted cells than the cap
+    (a request larger than the cap runs alone), every request completes, and a
+    request's fields are those of the uncapped server."""
+    sizes = [((32, 32, 32), 11, 12), ((32, 32, 32), 12, 9), ((40, 32, 32), 13, 14), ((32, 32, 32), 14, 7), ((64, 64, 64), 15, 6),
+             ((32, 32, 32), 16, 10)]
+    cap = 3 * 32 ** 3
+    out = {}
+    for limit in (0, cap):
+        with firesim.Server(hbm_budget_bytes=8 * GIB, sor_kernel=2, max_round_voxels=limit) as sv:
+            ids = [sv.submit(request(n, seed, steps, keep=(i == 2))) for i, (n, seed, steps) in enumerate(sizes)]
+            sv.run()
+            assert all(sv.status(i)["state"] == firesim.REQ_DONE for i in ids)
+            if limit:
+                assert sv.stats()["max_active_seen"] <= 3
+            s = sv.take_scene(ids[2])
+            out[limit] = {f: s.read(f) for f in FIELDS}
+            s.close()
+    for f in FIELDS:
+        assert np.array_equal(out[0][f], out[cap][f]), f
