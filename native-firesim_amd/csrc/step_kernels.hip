@@ -416,9 +416,62 @@ __global__ __launch_bounds__(256) void k_finalize_release(const double* __restri
 // integer-hash value noise of the edge's normalized position, drifting with
 // time, scaled so that its curl is of order 1.
 constexpr float kCurlFrequency = 6.0f;  // lattice cells across the domain
-__device__ __forceinline__ float curl_potential(int c, float ex, float ey, float ez, float t, const DevParams& p) {
+// The noise's lattice values at this step, shared by a workgroup. Edge
+// positions e in [0, 1] map to e * 6 + drift, so lattice indices run from
+// floor(drift) (e = 0 gives the drift exactly) to at most floor(drift) + 7
+// (6 cells, plus one when the FP32 sum rounds up to the next integer), and
+// the upper corners one further: 9 nodes per axis. Every workgroup hashes the
+// 3 x 9^3 nodes once (about ten hashes a thread, against 42 per cell when each
+// edge hashes its own eight corners); the values are those vnoise3 would hash.
+constexpr int kCurlLat = 9;
+struct CurlLattice {
+    float v[3][kCurlLat * kCurlLat * kCurlLat];
+    uint32_t row[3][kCurlLat * kCurlLat];
+};
+__device__ __forceinline__ int3 curl_lattice_origin(float t) {
+    return make_int3(static_cast<int>(floorf(t * 0.23f)), static_cast<int>(floorf(-(t * 0.17f))), static_cast<int>(floorf(t * 0.19f)));
+}
+// Called by every thread of the workgroup (it synchronizes).
+__device__ void build_curl_lattice(CurlLattice& L, const DevParams& p, float t) {
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nt = blockDim.x * blockDim.y * blockDim.z;
+    const int3 o = curl_lattice_origin(t);
+    __syncthreads();  // a previous use of L by this workgroup has finished
+    for (int i = tid; i < 3 * kCurlLat * kCurlLat; i += nt) {
+        const int c = i / (kCurlLat * kCurlLat), r = i % (kCurlLat * kCurlLat);
+        const uint32_t z = pcg(p.saltCurl[c].salt + static_cast<uint32_t>(o.z + r / kCurlLat));
+        L.row[c][r] = pcg(z + static_cast<uint32_t>(o.y + r % kCurlLat));
+    }
+    __syncthreads();
+    for (int i = tid; i < 3 * kCurlLat * kCurlLat * kCurlLat; i += nt) {
+        const int c = i / (kCurlLat * kCurlLat * kCurlLat), r = i % (kCurlLat * kCurlLat * kCurlLat);
+        const int zy = r / kCurlLat, ix = o.x + r % kCurlLat;
+        L.v[c][r] = p.saltCurl[c].sinHash
+                        ? sin_hash31(static_cast<float>(ix), static_cast<float>(o.y + zy % kCurlLat), static_cast<float>(o.z + zy / kCurlLat))
+                        : unit24(pcg(L.row[c][zy] + static_cast<uint32_t>(ix)));
+    }
+    __syncthreads();
+}
+// vnoise3 of component c from the lattice table (same corners, same blend).
+__device__ __forceinline__ float curl_potential(int c, float ex, float ey, float ez, float t, const DevParams& p, const CurlLattice& L) {
     const float scale = static_cast<float>(p.nx + p.ny + p.nz) / (3.0f * kCurlFrequency);
-    return scale * (vnoise3(ex * kCurlFrequency + t * 0.23f, ey * kCurlFrequency - t * 0.17f, ez * kCurlFrequency + t * 0.19f, p.saltCurl[c]) - 0.5f);
+    const float px = ex * kCurlFrequency + t * 0.23f, py = ey * kCurlFrequency - t * 0.17f, pz = ez * kCurlFrequency + t * 0.19f;
+    const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
+    const int3 o = curl_lattice_origin(t);
+    const int ix = min(max(static_cast<int>(fx0) - o.x, 0), kCurlLat - 2);
+    const int iy = min(max(static_cast<int>(fy0) - o.y, 0), kCurlLat - 2);
+    const int iz = min(max(static_cast<int>(fz0) - o.z, 0), kCurlLat - 2);
+    const float fx = px - fx0, fy = py - fy0, fz = pz - fz0;
+    const float ux = fx * fx * (3.0f - 2.0f * fx);
+    const float uy = fy * fy * (3.0f - 2.0f * fy);
+    const float uz = fz * fz * (3.0f - 2.0f * fz);
+    const float* v = L.v[c] + (iz * kCurlLat + iy) * kCurlLat + ix;
+    constexpr int DY = kCurlLat, DZ = kCurlLat * kCurlLat;
+    const float c00 = lerpf(v[0], v[1], ux);
+    const float c10 = lerpf(v[DY], v[DY + 1], ux);
+    const float c01 = lerpf(v[DZ], v[DZ + 1], ux);
+    const float c11 = lerpf(v[DZ + DY], v[DZ + DY + 1], ux);
+    return scale * (lerpf(lerpf(c00, c10, uy), lerpf(c01, c11, uy), uz) - 0.5f);
 }
 // Arrays of nx * (ny + 1) * nz edges, x fastest; edges at i = nx or k = nz are
 // wall edges (0) and are not stored.
@@ -431,17 +484,17 @@ __device__ __forceinline__ float psi_at(const float* __restrict__ psi, const Dev
 // Cell (x, y, z) writes the three edges it owns; the top row of cells also
 // writes the edges of the top face (j = ny).
 __device__ __forceinline__ void write_curl_potential(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz, const DevParams& p, int x,
-                                                     int y, int z, float t) {
+                                                     int y, int z, float t, const CurlLattice& L) {
     const float fx = static_cast<float>(x) / static_cast<float>(p.nx), cx = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
     const float fy = static_cast<float>(y) / static_cast<float>(p.ny), cy = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
     const float fz = static_cast<float>(z) / static_cast<float>(p.nz), cz = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
     const bool topWall = p.closedTop != 0;
-    st_at(px, psi_index(p, x, y, z), (y == 0 || z == 0) ? 0.0f : curl_potential(0, cx, fy, fz, t, p));
-    st_at(py, psi_index(p, x, y, z), (x == 0 || z == 0) ? 0.0f : curl_potential(1, fx, cy, fz, t, p));
-    st_at(pz, psi_index(p, x, y, z), (x == 0 || y == 0) ? 0.0f : curl_potential(2, fx, fy, cz, t, p));
+    st_at(px, psi_index(p, x, y, z), (y == 0 || z == 0) ? 0.0f : curl_potential(0, cx, fy, fz, t, p, L));
+    st_at(py, psi_index(p, x, y, z), (x == 0 || z == 0) ? 0.0f : curl_potential(1, fx, cy, fz, t, p, L));
+    st_at(pz, psi_index(p, x, y, z), (x == 0 || y == 0) ? 0.0f : curl_potential(2, fx, fy, cz, t, p, L));
     if (y == p.ny - 1) {
-        st_at(px, psi_index(p, x, p.ny, z), (topWall || z == 0) ? 0.0f : curl_potential(0, cx, 1.0f, fz, t, p));
-        st_at(pz, psi_index(p, x, p.ny, z), (topWall || x == 0) ? 0.0f : curl_potential(2, fx, 1.0f, cz, t, p));
+        st_at(px, psi_index(p, x, p.ny, z), (topWall || z == 0) ? 0.0f : curl_potential(0, cx, 1.0f, fz, t, p, L));
+        st_at(pz, psi_index(p, x, p.ny, z), (topWall || x == 0) ? 0.0f : curl_potential(2, fx, 1.0f, cz, t, p, L));
     }
 }
 
@@ -449,7 +502,7 @@ template <bool B>
 __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, float* __restrict__ psix,
                                                  float* __restrict__ psiy, float* __restrict__ psiz, const Scalars<B>& S,
                                                  const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                                                 const DevParams& p, int x, int y, int z) {
+                                                 const DevParams& p, const CurlLattice& L, int x, int y, int z) {
     // The two noise fields first, by every lane of the wavefront (lanes are
     // consecutive x cells of one row, so noiseA's lattice row and noiseB's
     // lattice column are shared by the whole wavefront). With curl noise on,
@@ -465,7 +518,7 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
         noiseB = fbm2_t<2>(w * 10.0f - t * 0.19f, u * 8.0f + v * 5.5f + t * 0.27f, p.saltNoise2) - 0.5f;
     }
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
-    if (curl) write_curl_potential(psix, psiy, psiz, p, x, y, z, t);
+    if (curl) write_curl_potential(psix, psiy, psiz, p, x, y, z, t, L);
     const Idx idx = sidx(p, x, y, z);
     const int sid = p.sceneId;
     const float h = S.template ld<HEAT>(idx), s = S.template ld<SOOT>(idx), k = S.template ld<TURB>(idx), c = S.template ld<PROGRESS>(idx);
@@ -515,8 +568,10 @@ __global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, f
             acc[1] += sum;
         }
     }
+    __shared__ CurlLattice lat;
+    if (p.curlOn) build_curl_lattice(lat, p, ld_at(p.clock, 0u));
     const int3 c = cell_of(xcd_block());
-    force_contrib_at<B>(cu, cv, cw, psix, psiy, psiz, S, U, V, W, p, c.x, c.y, c.z);
+    force_contrib_at<B>(cu, cv, cw, psix, psiy, psiz, S, U, V, W, p, lat, c.x, c.y, c.z);
 }
 
 // New face values after the gathered force and the boundary clamp
@@ -999,10 +1054,13 @@ __global__ __launch_bounds__(256) void kb_finalize_release(const BatchTable* __r
 // Held to eight wavefronts per SIMD as kb_advect_velocity (67 -> 64 VGPRs, no spills).
 template <bool B>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void kb_force_contrib(const BatchTable* __restrict__ T) {
+    __shared__ CurlLattice lat;
     batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
+        if (S.p.curlOn) build_curl_lattice(lat, S.p, ld_at(S.p.clock, 0u));
         const int3 c = cell_of(block_of(local, S.cellG));
-        force_contrib_at<B>(S.cu, S.cv, S.cw, S.psi[0], S.psi[1], S.psi[2], scal_of<B>(S.sOut), S.velB[0], S.velB[1], S.velB[2], S.p, c.x, c.y, c.z);
+        force_contrib_at<B>(S.cu, S.cv, S.cw, S.psi[0], S.psi[1], S.psi[2], scal_of<B>(S.sOut), S.velB[0], S.velB[1], S.velB[2], S.p, lat, c.x, c.y,
+                            c.z);
     });
 }
 
