@@ -144,6 +144,10 @@ int batch_grid(int device);
 // Grid and block shape of the cell-parallel passes (64 lanes along x).
 dim3 cell_block();
 dim3 cell_grid(int nx, int ny, int nz);
+// z planes of a force-contribution workgroup (64 x 4 x kForcePlanes cells,
+// one curl-noise lattice build each), and that kernel's grid.
+constexpr int kForcePlanes = 4;
+dim3 force_grid(int nx, int ny, int nz);
 dim3 light_block();
 dim3 light_grid(int lnx, int lny, int lnz);
 dim3 sor_grid(int sorKernel, int nx, int ny, int nz);

@@ -551,19 +551,33 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
     st_at(cw, idx, p.dt * (p.windZ * windProfile + (gradX * 1.36f + noiseB * (0.20f + upperDrag * 0.16f) + curlZ * vortexGain) * curlGain));
 }
 
+// The force-contribution kernel's grid of 64 x 4 x kForcePlanes cell blocks.
+dim3 force_grid(int nx, int ny, int nz) {
+    const dim3 b = cell_block();
+    return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y, (nz + kForcePlanes - 1) / kForcePlanes);
+}
+
 // With a non-null `partial`, workgroup (0, 0, 0) first sums the react
-// partials as k_finalize_release does (same order, same result), hidden
-// behind the other workgroups of this launch.
+// partials as k_finalize_release does (same order, same result: its first 256
+// threads are that kernel's four wavefronts), hidden behind the other
+// workgroups of this launch. Workgroups are 64 x 4 x kForcePlanes cells, so
+// the curl-noise lattice is built once for 1024 cells.
 template <bool B>
-__global__ __launch_bounds__(256) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, float* __restrict__ psix,
-                                                       float* __restrict__ psiy, float* __restrict__ psiz, Scalars<B> S,
-                                                       const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                                                       DevParams p, const double* __restrict__ partial, size_t npartial,
-                                                       double* __restrict__ acc) {
+__global__ __launch_bounds__(1024) void k_force_contrib(float* __restrict__ cu, float* __restrict__ cv, float* __restrict__ cw, float* __restrict__ psix,
+                                                        float* __restrict__ psiy, float* __restrict__ psiz, Scalars<B> S,
+                                                        const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                                        DevParams p, const double* __restrict__ partial, size_t npartial,
+                                                        double* __restrict__ acc) {
     if (partial != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) {
-        const unsigned t = threadIdx.x + blockDim.x * threadIdx.y;
-        const double sum = block_sum_256(strided_sum_256(partial, npartial, t));
+        __shared__ double red[4];
+        const unsigned t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+        if (t < 256) {
+            const double v = wave_sum(strided_sum_256(partial, npartial, t));
+            if ((t & 63) == 0) red[t >> 6] = v;
+        }
+        __syncthreads();
         if (t == 0) {
+            const double sum = (red[0] + red[1]) + (red[2] + red[3]);
             acc[0] = sum;
             acc[1] += sum;
         }
@@ -622,15 +636,39 @@ __device__ __forceinline__ double face_w(const float* W, const float* cw, const 
     return fmin(4.0, fmax(-4.0, val));
 }
 
-__device__ __forceinline__ void force_apply_at(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
-                                               const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
-                                               const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
-                                               const CurlPsi& psi, float* __restrict__ D, float* __restrict__ P, const DevParams& p, int x, int y,
-                                               int z) {
-    if (x >= p.nx || y >= p.ny || z >= p.nz) return;
-    const double uL = face_u(U, cu, psi, p, x, y, z), uR = face_u(U, cu, psi, p, x + 1, y, z);
-    const double vD = face_v(V, cv, psi, p, x, y, z), vU = face_v(V, cv, psi, p, x, y + 1, z);
-    const double wB = face_w(W, cw, psi, p, x, y, z), wF = face_w(W, cw, psi, p, x, y, z + 1);
+// The force pass's output for a 64 x 4 block of one plane (thread (tx, ty)
+// owns x = x0 + tx, y = y0 + ty): each cell stores its u, v and w faces (and
+// the wall faces past the domain's last column, row and plane), its
+// divergence -(uR - uL + vU - vD + wF - wB) from the unrounded face values, and
+// p = 0 (the divergence kernel :1185-1199). The u and v faces are computed
+// once: a cell's u face x + 1 is the next lane's u face x and its v face y + 1
+// the v face y of the row above (through LDS); lane 63, the top row of the
+// block and the domain's last column and row compute those faces themselves.
+// Called by every thread of the workgroup (it synchronizes); vrow holds 256
+// doubles and may be reused on return.
+__device__ __forceinline__ void force_apply_block(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
+                                                  const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+                                                  const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
+                                                  const CurlPsi& psi, float* __restrict__ D, float* __restrict__ P, const DevParams& p, uint3 blk,
+                                                  double* vrow) {
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = static_cast<int>(blk.x) * 64 + tx, y = static_cast<int>(blk.y) * 4 + ty, z = static_cast<int>(blk.z);
+    const bool in = x < p.nx && y < p.ny && z < p.nz;
+    double uL = 0.0, vD = 0.0, wB = 0.0, wF = 0.0;
+    if (in) {
+        uL = face_u(U, cu, psi, p, x, y, z);
+        vD = face_v(V, cv, psi, p, x, y, z);
+        wB = face_w(W, cw, psi, p, x, y, z);
+        wF = face_w(W, cw, psi, p, x, y, z + 1);
+    }
+    double uR = __shfl_down(uL, 1, 64);
+    vrow[ty * 64 + tx] = vD;
+    __syncthreads();
+    double vU = ty < 3 ? vrow[(ty + 1) * 64 + tx] : 0.0;
+    __syncthreads();
+    if (!in) return;
+    if (tx == 63 || x == p.nx - 1) uR = face_u(U, cu, psi, p, x + 1, y, z);
+    if (ty == 3 || y == p.ny - 1) vU = face_v(V, cv, psi, p, x, y + 1, z);
     st_at(uo, uidx(p, x, y, z), static_cast<float>(uL));
     if (x == p.nx - 1) st_at(uo, uidx(p, x + 1, y, z), static_cast<float>(uR));
     st_at(vo, vidx(p, x, y, z), static_cast<float>(vD));
@@ -647,8 +685,8 @@ __global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, flo
                                                      const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
                                                      const float* __restrict__ cu, const float* __restrict__ cv, const float* __restrict__ cw,
                                                      CurlPsi psi, float* __restrict__ D, float* __restrict__ P, DevParams p) {
-    const int3 c = cell_of(xcd_block());
-    force_apply_at(uo, vo, wo, U, V, W, cu, cv, cw, psi, D, P, p, c.x, c.y, c.z);
+    __shared__ double vrow[256];
+    force_apply_block(uo, vo, wo, U, V, W, cu, cv, cw, psi, D, P, p, xcd_block(), vrow);
 }
 
 // -------------------------------------------------------------- K12 ----
@@ -744,46 +782,71 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
     // local index range of real cells
     const int rx0 = max(0, -bx), ry0 = max(0, -by), rz0 = max(0, -bz);
     const int rx1 = min(E, nx - bx), ry1 = min(E, ny - by), rz1 = min(EZ, nz - bz);
-    const int k = tid % HX, ly = (tid / HX) % E, zg = tid / (HX * E);
+    const int k = tid % HX, ly = (tid / HX) % E;
+    // The z group is the same for a whole wavefront when E^2/2 threads fill
+    // whole wavefronts: the plane loop then runs on scalar registers.
+    int zg = tid / (HX * E);
+    if constexpr ((HX * E) % 64 == 0) zg = __builtin_amdgcn_readfirstlane(zg);
     const int gy = by + ly;
+    // Per colour parity: the thread's x in the box, its LDS offset within a
+    // plane, its d offset within a plane, and its wall-copy offset (the slot
+    // past the wall of an x-wall cell, else 0).
+    const int c1 = (bx + gy + bz + zg) & 1;
+    int lxP[2], iP[2], cxP[2];
+    Idx dP[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        lxP[q] = 2 * k + (q ^ c1);
+        const int gx = bx + lxP[q];
+        iP[q] = ly * E + lxP[q];
+        dP[q] = static_cast<Idx>(gy) * row + static_cast<Idx>(gx);
+        cxP[q] = gx == 0 ? -1 : (gx == nx - 1 ? 1 : 0);
+    }
+    const int cy = gy == 0 ? -E : (gy == ny - 1 ? E : 0);
+    // Boxes that reach a domain wall in x or y carry wall cells; their sweeps
+    // also write each updated cell to its wall-copy slot (the same slot again
+    // for cells off the wall), so no separate refresh pass is needed.
+    const bool wallX = bx < 0 || bx + E >= nx, wallY = by < 0 || by + E >= ny;
     for (int s = 0; s < count; ++s) {
-        const int parity = (h0 + s) & 1;
+        const int q = (h0 + s) & 1;
         const int ax = bx < 0 ? rx0 : s + 1, ay = by < 0 ? ry0 : s + 1, az = bz < 0 ? rz0 : s + 1;
         const int ex = ox + T + K > nx ? rx1 : E - 1 - s;
         const int ey = oy + T + K > ny ? ry1 : E - 1 - s;
         const int ez = oz + TZ + K > nz ? rz1 : EZ - 1 - s;
-        const int lx = 2 * k + ((parity ^ (bx + gy + bz + zg)) & 1);
-        if (lx >= ax && lx < ex && ly >= ay && ly < ey) {
-            const int gx = bx + lx;
-            const int lz0 = az + ((az ^ zg) & 1);
-            const int i0 = (lz0 * E + ly) * E + lx;
+        const int lx = lxP[q];
+        const int lz0 = az + ((az ^ zg) & 1);
+        if (lx >= ax && lx < ex && ly >= ay && ly < ey && lz0 < ez) {
+            const int i0 = lz0 * NT + iP[q];
+            const int n = (ez - lz0 + 1) >> 1;  // planes lz0, lz0 + 2, ... below ez
+            const int cx = cxP[q];
+            Idx di = static_cast<Idx>(bz + lz0) * plane + dP[q];
             int i = i0;
-            Idx di = static_cast<Idx>(bz + lz0) * plane + static_cast<Idx>(gy) * row + static_cast<Idx>(gx);
             // The cell above plane lz (other colour, unchanged in this
             // half-sweep) is the cell below plane lz + 2: read it once.
-            float pB = lz0 < ez ? sp[i0 - NT] : 0.0f;
+            float pB = sp[i0 - NT];
+            float v = 0.0f;
+            auto sweep = [&](auto wx, auto wy) {
 #pragma unroll UNR
-            for (int lz = lz0; lz < ez; lz += 2, i += 2 * NT, di += 2 * plane) {
-                const float pF = sp[i + NT];
-                sp[i] = sor_update(ld_at(D, di), sp[i], sp[i - 1], sp[i + 1], sp[i - E], sp[i + E], pB, pF, omega);
-                pB = pF;
-            }
-            // Wall cells refresh the slot past their wall, which only they
-            // read (in a later half-sweep).
-            const int gxo = gx == 0 ? -1 : (gx == nx - 1 ? 1 : 0);
-            const int gyo = gy == 0 ? -E : (gy == ny - 1 ? E : 0);
-            if (gxo != 0 || gyo != 0) {
-                for (int j = i0; j < i; j += 2 * NT) {
-                    const float r = sp[j];
-                    if (gxo != 0) sp[j + gxo] = r;
-                    if (gyo != 0) sp[j + gyo] = r;
+                for (int j = 0; j < n; ++j, i += 2 * NT, di += 2 * plane) {
+                    const float pF = sp[i + NT];
+                    v = sor_update(ld_at(D, di), sp[i], sp[i - 1], sp[i + 1], sp[i - E], sp[i + E], pB, pF, omega);
+                    sp[i] = v;
+                    if constexpr (decltype(wx)::value) sp[i + cx] = v;
+                    if constexpr (decltype(wy)::value) sp[i + cy] = v;
+                    pB = pF;
                 }
+            };
+            if (wallX) {
+                if (wallY) sweep(std::true_type{}, std::true_type{});
+                else sweep(std::true_type{}, std::false_type{});
+            } else {
+                if (wallY) sweep(std::false_type{}, std::true_type{});
+                else sweep(std::false_type{}, std::false_type{});
             }
-            if (i > i0) {
-                if (bz + lz0 == 0) sp[i0 - NT] = sp[i0];
-                const int il = i - 2 * NT;  // last plane updated
-                if (bz + lz0 + (il - i0) / NT == nz - 1) sp[il + NT] = sp[il];
-            }
+            // z-wall cells refresh the slot past their wall, which only they
+            // read (in a later half-sweep).
+            if (bz + lz0 == 0) sp[i0 - NT] = sp[i0];
+            if (bz + lz0 + 2 * (n - 1) == nz - 1) sp[i0 + (2 * n - 1) * NT] = v;
         }
         __syncthreads();
     }
@@ -1052,12 +1115,19 @@ __global__ __launch_bounds__(256) void kb_finalize_release(const BatchTable* __r
 }
 
 // Held to eight wavefronts per SIMD as kb_advect_velocity (67 -> 64 VGPRs, no spills).
+// A workgroup's consecutive blocks mostly belong to one scene: the curl-noise
+// lattice (a function of the scene's salts and clock) is rebuilt only when
+// the scene changes.
 template <bool B>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void kb_force_contrib(const BatchTable* __restrict__ T) {
+__global__ __launch_bounds__(256) void kb_force_contrib(const BatchTable* __restrict__ T) {
     __shared__ CurlLattice lat;
+    int built = -1;
     batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
-        if (S.p.curlOn) build_curl_lattice(lat, S.p, ld_at(S.p.clock, 0u));
+        if (S.p.curlOn && s != built) {
+            build_curl_lattice(lat, S.p, ld_at(S.p.clock, 0u));
+            built = s;
+        }
         const int3 c = cell_of(block_of(local, S.cellG));
         force_contrib_at<B>(S.cu, S.cv, S.cw, S.psi[0], S.psi[1], S.psi[2], scal_of<B>(S.sOut), S.velB[0], S.velB[1], S.velB[2], S.p, lat, c.x, c.y,
                             c.z);
@@ -1065,11 +1135,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 }
 
 __global__ __launch_bounds__(256) void kb_force_apply(const BatchTable* __restrict__ T) {
+    __shared__ double vrow[256];
     batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
-        const int3 c = cell_of(block_of(local, S.cellG));
-        force_apply_at(S.velA[0], S.velA[1], S.velA[2], S.velB[0], S.velB[1], S.velB[2], S.cu, S.cv, S.cw, CurlPsi{S.psi[0], S.psi[1], S.psi[2]}, S.d,
-                       S.p0, S.p, c.x, c.y, c.z);
+        force_apply_block(S.velA[0], S.velA[1], S.velA[2], S.velB[0], S.velB[1], S.velB[2], S.cu, S.cv, S.cw, CurlPsi{S.psi[0], S.psi[1], S.psi[2]},
+                          S.d, S.p0, S.p, block_of(local, S.cellG), vrow);
     });
 }
 
@@ -1143,12 +1213,12 @@ hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t s
 }
 
 hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalizeRelease) {
-    const dim3 g = cell_grid(s.nx, s.ny, s.nz), b = cell_block();
+    const dim3 g = force_grid(s.nx, s.ny, s.nz), b = cell_block(), bc(64, 4, kForcePlanes);
     const double* part = finalizeRelease ? s.partials : nullptr;
     const size_t np = react_partial_count(s.nx, s.ny, s.nz);
-    if (s.bf16) k_force_contrib<true><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, s.psi[0], s.psi[1], s.psi[2], scal<true>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
-    else k_force_contrib<false><<<g, b, 0, st>>>(s.cu, s.cv, s.cw, s.psi[0], s.psi[1], s.psi[2], scal<false>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
-    k_force_apply<<<g, b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, s.cu, s.cv, s.cw, CurlPsi{s.psi[0], s.psi[1], s.psi[2]}, s.d, s.p, p);
+    if (s.bf16) k_force_contrib<true><<<g, bc, 0, st>>>(s.cu, s.cv, s.cw, s.psi[0], s.psi[1], s.psi[2], scal<true>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
+    else k_force_contrib<false><<<g, bc, 0, st>>>(s.cu, s.cv, s.cw, s.psi[0], s.psi[1], s.psi[2], scal<false>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
+    k_force_apply<<<cell_grid(s.nx, s.ny, s.nz), b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, s.cu, s.cv, s.cw, CurlPsi{s.psi[0], s.psi[1], s.psi[2]}, s.d, s.p, p);
     std::swap(s.u, s.uN);
     std::swap(s.v, s.vN);
     std::swap(s.w, s.wN);
