@@ -29,6 +29,7 @@ constexpr float kAdvectionScale = 0.62f;  // src/fire_cuda.cu:107
 // Per-pass parameter block (SimParams src/fire_cuda.cu:16-68), passed by value.
 struct DevParams {
     int nx, ny, nz;
+    float invN[3];  // 1/nx, 1/ny, 1/nz: normalized coordinates are index * invN (within one ulp of index / n)
     int lnx, lny, lnz;
     int fw, fh;
     float dt;
@@ -142,12 +143,15 @@ __device__ __forceinline__ F3 normalize3(F3 v) {
     return f3(v.x / len, v.y / len, v.z / len);
 }
 
-// Smoothstep with inverted edges (src/fire_cuda.cu:227-234).
+// Smoothstep with inverted edges (src/fire_cuda.cu:227-234). The quotient is
+// formed with the reciprocal of the edge span: with constant edges (nearly
+// every call) that is one multiply by a folded constant instead of a full
+// FP32 division, at most one ulp apart.
 __device__ __forceinline__ float sstep(float e0, float e1, float x) {
     const bool inv = e1 < e0;
     const float lo = inv ? e1 : e0;
     const float hi = inv ? e0 : e1;
-    const float t = sat((x - lo) / fmaxf(0.000001f, hi - lo));
+    const float t = sat((x - lo) * (1.0f / fmaxf(0.000001f, hi - lo)));
     const float v = t * t * (3.0f - 2.0f * t);
     return inv ? 1.0f - v : v;
 }

@@ -213,6 +213,9 @@ DevParams dev_params(const fs_scene& sc) {
     const fs_params& s = sc.prm;
     const Dims& d = sc.dims;
     p.nx = d.nx; p.ny = d.ny; p.nz = d.nz;
+    p.invN[0] = 1.0f / static_cast<float>(d.nx);
+    p.invN[1] = 1.0f / static_cast<float>(d.ny);
+    p.invN[2] = 1.0f / static_cast<float>(d.nz);
     p.lnx = d.lnx; p.lny = d.lny; p.lnz = d.lnz;
     p.fw = d.fw; p.fh = d.fh;
     p.dt = s.dt;

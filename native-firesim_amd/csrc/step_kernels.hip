@@ -94,9 +94,9 @@ __device__ __forceinline__ void advect_velocity_at(float* __restrict__ uo, float
     const float k = p.dt * kAdvectionScale;
     // remainders of the face (a) and centre (c) coordinates on the
     // cell-aligned axes; on face-aligned axes they are 0 and 0.5
-    const float axr = -static_cast<float>(x) / static_cast<float>(nx), cxr = 0.5f - (static_cast<float>(x) + 0.5f) / static_cast<float>(nx);
-    const float ayr = -static_cast<float>(y) / static_cast<float>(ny), cyr = 0.5f - (static_cast<float>(y) + 0.5f) / static_cast<float>(ny);
-    const float azr = -static_cast<float>(z) / static_cast<float>(nz), czr = 0.5f - (static_cast<float>(z) + 0.5f) / static_cast<float>(nz);
+    const float axr = -static_cast<float>(x) * p.invN[0], cxr = 0.5f - (static_cast<float>(x) + 0.5f) * p.invN[0];
+    const float ayr = -static_cast<float>(y) * p.invN[1], cyr = 0.5f - (static_cast<float>(y) + 0.5f) * p.invN[1];
+    const float azr = -static_cast<float>(z) * p.invN[2], czr = 0.5f - (static_cast<float>(z) + 0.5f) * p.invN[2];
     const Idx rowU = static_cast<Idx>(nx + 1), planeU = rowU * static_cast<Idx>(ny);
     const Idx rowV = static_cast<Idx>(nx), planeV = rowV * static_cast<Idx>(ny + 1);
     const Idx rowW = static_cast<Idx>(nx), planeW = rowW * static_cast<Idx>(ny);
@@ -193,9 +193,9 @@ template <bool B>
 __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const float* __restrict__ U, const float* __restrict__ V,
                             const float* __restrict__ W, const DevParams& p, int x, int y, int z) {
     const Idx idx = sidx(p, x, y, z);
-    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
-    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
-    const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const float u = (static_cast<float>(x) + 0.5f) * p.invN[0];
+    const float v = (static_cast<float>(y) + 0.5f) * p.invN[1];
+    const float w = (static_cast<float>(z) + 0.5f) * p.invN[2];
     const float dt = p.dt;
     const float time = ld_at(p.clock, 0u);
     const int sid = p.sceneId;
@@ -225,14 +225,28 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
         out.template st<SOPT>(idx, 0.0f);
         return 0.0f;
     }
-    // backtrace and forward-trace positions in double (see tri_setup)
-    const double du = (x + 0.5) / static_cast<double>(p.nx), dv = (y + 0.5) / static_cast<double>(p.ny), dw = (z + 0.5) / static_cast<double>(p.nz);
-    const double k = static_cast<double>(dt) * static_cast<double>(kAdvectionScale);
-    const F3 vel = sample_vel<double>(U, V, W, p, du, dv, dw);
-    const double pu = du - vel.x * k, pv = dv - vel.y * k, pw = dw - vel.z * k;
-    const F3 pvel = sample_vel<double>(U, V, W, p, pu, pv, pw);
-    const Tri tp = tri_scalar_setup<double>(p, pu, pv, pw);
-    const Tri tf = tri_scalar_setup<double>(p, pu + pvel.x * k, pv + pvel.y * k, pw + pvel.z * k);
+    // Backtrace and forward trace in index units, as advect_velocity_at: the
+    // normalized centre (x + 1/2)/n maps to x + 1/2 on a face-aligned axis
+    // (scale n) and to x + r on a cell-aligned one (scale n - 1, r = 1/2 -
+    // (x + 1/2)/n); a displacement d in normalized units adds -d * scale to
+    // the remainder. The forward trace starts from the unclamped backtrace.
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const float k = dt * kAdvectionScale;
+    const float cxr = 0.5f - u, cyr = 0.5f - v, czr = 0.5f - w;
+    const float sxF = static_cast<float>(nx), syF = static_cast<float>(ny), szF = static_cast<float>(nz);
+    const float sxC = static_cast<float>(nx - 1), syC = static_cast<float>(ny - 1), szC = static_cast<float>(nz - 1);
+    const Axis cxC = axis_at(nx, nx - 1, x, cxr), cyC = axis_at(ny, ny - 1, y, cyr), czC = axis_at(nz, nz - 1, z, czr);
+    const float vx = tri_plain(U, tri_of(axis_at(nx + 1, nx, x, 0.5f), cyC, czC, nx + 1, ny));
+    const float vy = tri_plain(V, tri_of(cxC, axis_at(ny + 1, ny, y, 0.5f), czC, nx, ny + 1));
+    const float vz = tri_plain(W, tri_of(cxC, cyC, axis_at(nz + 1, nz, z, 0.5f), nx, ny));
+    const float bxC = cxr - vx * k * sxC, byC = cyr - vy * k * syC, bzC = czr - vz * k * szC;
+    const Axis pxC = axis_at(nx, nx - 1, x, bxC), pyC = axis_at(ny, ny - 1, y, byC), pzC = axis_at(nz, nz - 1, z, bzC);
+    const float px = tri_plain(U, tri_of(axis_at(nx + 1, nx, x, 0.5f - vx * k * sxF), pyC, pzC, nx + 1, ny));
+    const float py = tri_plain(V, tri_of(pxC, axis_at(ny + 1, ny, y, 0.5f - vy * k * syF), pzC, nx, ny + 1));
+    const float pz = tri_plain(W, tri_of(pxC, pyC, axis_at(nz + 1, nz, z, 0.5f - vz * k * szF), nx, ny));
+    const Tri tp = tri_of(pxC, pyC, pzC, nx, ny);
+    const Tri tf = tri_of(axis_at(nx, nx - 1, x, bxC + px * k * sxC), axis_at(ny, ny - 1, y, byC + py * k * syC),
+                          axis_at(nz, nz - 1, z, bzC + pz * k * szC), nx, ny);
     const float anti = sat(0.30f + p.detail * 0.34f);
     float heat = maccormack<HEAT, B>(in, tp, tf, h0, anti);
     float fuel = maccormack<FUEL, B>(in, tp, tf, f0, anti * 0.92f);
@@ -266,7 +280,7 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
     float tempK = ambientK + heat * 335.0f;
     const float radFb = sstep(430.0f, 980.0f, tempK);
     const float solidO2 = sat(oxygen * 1.28f - soot * 0.030f);
-    const float heatFlux = sat((tempK - 405.0f) / 900.0f) + bed * 0.34f + sootOptics * 0.024f;
+    const float heatFlux = sat((tempK - 405.0f) * (1.0f / 900.0f)) + bed * 0.34f + sootOptics * 0.024f;
     const float charRelease = fminf(charMass, charMass * heatFlux * solidO2 * (0.14f + radFb * 1.38f) * dt);
     pyro += (bed * (0.34f + radFb * 1.05f) * dt + charRelease * (2.45f + sourceNoise * 0.72f)) * pyroGain;
     charMass += material * dt * (sid == 2 ? 0.00005f : (sid == 1 ? 0.050f : 0.026f));
@@ -278,7 +292,7 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
 
     if (p.leftDown != 0) {
         const float mx = p.mouseX, my = p.mouseY;
-        const float source = expf(-((u - mx) * (u - mx)) / 0.0180f - ((w - 0.5f) * (w - 0.5f)) / 0.0550f - ((v - my) * (v - my)) / 0.0020f);
+        const float source = expf(-((u - mx) * (u - mx)) * (1.0f / 0.0180f) - ((w - 0.5f) * (w - 0.5f)) * (1.0f / 0.0550f) - ((v - my) * (v - my)) * (1.0f / 0.0020f));
         heat += source * dt * 2.8f;
         fuel += source * dt * 1.4f;
         pyro += source * dt * 0.45f;
@@ -287,7 +301,7 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
         oxygen = fmaxf(oxygen, 0.72f + source * 0.20f);
     }
     if (p.rightDown != 0) {
-        const float source = expf(-((u - p.mouseX) * (u - p.mouseX) + (w - 0.5f) * (w - 0.5f)) / 0.0100f - (v - p.mouseY) * (v - p.mouseY) / 0.0100f);
+        const float source = expf(-((u - p.mouseX) * (u - p.mouseX) + (w - 0.5f) * (w - 0.5f)) * (1.0f / 0.0100f) - (v - p.mouseY) * (v - p.mouseY) * (1.0f / 0.0100f));
         soot += source * dt * 8.0f;
         heat *= 1.0f - source * dt * 1.2f;
         fuel *= 1.0f - source * dt * 0.7f;
@@ -323,7 +337,7 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
     oxygen += (1.0f - oxygen) * dt * 0.08f;
 
     tempK = ambientK + heat * 335.0f;
-    const float radBase = fmaxf(0.0f, (tempK - ambientK) / 1560.0f);
+    const float radBase = fmaxf(0.0f, (tempK - ambientK) * (1.0f / 1560.0f));
     const float radSq = radBase * radBase;
     heat -= radSq * radSq * dt * (0.88f + soot * 0.085f + sootOptics * 0.018f);
     heat *= expf(-dt * (0.31f + v * 0.64f + soot * 0.022f + bA * 0.26f + upper * 0.18f));
@@ -338,7 +352,7 @@ __device__ float react_cell(const Scalars<B>& out, const Scalars<B>& in, const f
     const float flameActivity = sat(progress * 0.64f + burnMass * 5.2f + heat * 0.065f);
     turb += dt * (shear * (0.72f + progress * 0.06f) + flameActivity * p.turbulence * 0.62f + front * 0.085f + soot * 0.018f);
     turb *= expf(-dt * (0.54f + v * 0.24f + bA * 0.42f + upper * 0.16f));
-    const float growth = sat(incomplete * 0.36f + ash * 0.13f + soot * 0.055f + sstep(0.65f, 1.55f, tempK / 1000.0f) * 0.18f);
+    const float growth = sat(incomplete * 0.36f + ash * 0.13f + soot * 0.055f + sstep(0.65f, 1.55f, tempK * (1.0f / 1000.0f)) * 0.18f);
     sootOptics = soot * (1.04f + 0.28f * sqrtf(fmaxf(0.0f, heat)) + 0.24f * ash + 0.46f * growth);
     sootOptics = lerpf(sootOptics, sootOptics * 0.48f, sat(sootOxidation * 0.92f));
 
@@ -485,9 +499,9 @@ __device__ __forceinline__ float psi_at(const float* __restrict__ psi, const Dev
 // writes the edges of the top face (j = ny).
 __device__ __forceinline__ void write_curl_potential(float* __restrict__ px, float* __restrict__ py, float* __restrict__ pz, const DevParams& p, int x,
                                                      int y, int z, float t, const CurlLattice& L) {
-    const float fx = static_cast<float>(x) / static_cast<float>(p.nx), cx = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
-    const float fy = static_cast<float>(y) / static_cast<float>(p.ny), cy = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
-    const float fz = static_cast<float>(z) / static_cast<float>(p.nz), cz = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const float fx = static_cast<float>(x) * p.invN[0], cx = (static_cast<float>(x) + 0.5f) * p.invN[0];
+    const float fy = static_cast<float>(y) * p.invN[1], cy = (static_cast<float>(y) + 0.5f) * p.invN[1];
+    const float fz = static_cast<float>(z) * p.invN[2], cz = (static_cast<float>(z) + 0.5f) * p.invN[2];
     const bool topWall = p.closedTop != 0;
     st_at(px, psi_index(p, x, y, z), (y == 0 || z == 0) ? 0.0f : curl_potential(0, cx, fy, fz, t, p, L));
     st_at(py, psi_index(p, x, y, z), (x == 0 || z == 0) ? 0.0f : curl_potential(1, fx, cy, fz, t, p, L));
@@ -507,9 +521,9 @@ __device__ __forceinline__ void force_contrib_at(float* __restrict__ cu, float* 
     // consecutive x cells of one row, so noiseA's lattice row and noiseB's
     // lattice column are shared by the whole wavefront). With curl noise on,
     // the fbm terms are replaced by the potential's curl on the faces.
-    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
-    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
-    const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const float u = (static_cast<float>(x) + 0.5f) * p.invN[0];
+    const float v = (static_cast<float>(y) + 0.5f) * p.invN[1];
+    const float w = (static_cast<float>(z) + 0.5f) * p.invN[2];
     const float t = ld_at(p.clock, 0u);
     const bool curl = p.curlOn;
     float noiseA = 0.0f, noiseB = 0.0f;
@@ -908,9 +922,9 @@ __global__ __launch_bounds__(256) void k_reset(Scalars<B> S, float* __restrict__
     const int z = blk.z * blockDim.z + threadIdx.z;
     if (x >= p.nx || y >= p.ny || z >= p.nz) return;
     const Idx idx = sidx(p, x, y, z);
-    const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(p.nx);
-    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(p.ny);
-    const float w = (static_cast<float>(z) + 0.5f) / static_cast<float>(p.nz);
+    const float u = (static_cast<float>(x) + 0.5f) * p.invN[0];
+    const float v = (static_cast<float>(y) + 0.5f) * p.invN[1];
+    const float w = (static_cast<float>(z) + 0.5f) * p.invN[2];
     const float wx = (u - 0.50f) * 2.10f, wz = (w - 0.50f) * 1.64f;
     const float material = bed_material(wx, wz, p);
     float heat = 0, fuel = 0, soot = 0, ch = 0, ash = 0, pyro = 0, prog = 0, k = 0, so = 0;
