@@ -1259,7 +1259,21 @@ def _line(px, py, ax, ay, bx, by, th):
     return float(ss(th, th * f(0.25), math.sqrt(float(d @ d))))
 
 
-def _room_reference(prm, light, shadow, cam, uv, rd, glow, time):
+def _dir_of(prm):
+    """The directional light's travel direction, normalized in FP32 (a zero
+    vector means straight down)."""
+    d = np.array([prm.dir_light_x, prm.dir_light_y, prm.dir_light_z], dtype=np.float32)
+    n = np.float32(np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])))
+    if not (n > np.float32(1e-6) and np.isfinite(n)):
+        return np.array([0.0, -1.0, 0.0])
+    return (d / n).astype(np.float64)
+
+
+def _dir_colour(prm):
+    return np.array([f(prm.dir_light_r), f(prm.dir_light_g), f(prm.dir_light_b)])
+
+
+def _room_reference(prm, light, shadow, cam, uv, rd, glow, time, dirT=None):
     """roomBackgroundRay (:1683-1865) with gatherSceneIrradiance (:1589-1626),
     volumeShadowRay (:1628-1645) and sourceModelOverlay (:1661-1681);
     floorGrid (:1567-1573) is identically 0."""
@@ -1396,6 +1410,12 @@ def _room_reference(prm, light, shadow, cam, uv, rd, glow, time):
     lift = f(0.66) + float(ss(f(0.006), f(0.13), lum(*indirect))) * f(0.72)
     warm = f(1.72) if surf == 1 else (f(0.88) if surf == 2 else f(1.16))
     col = col + (indirect * alb) * (gis * fall * cocc * lift * warm)
+    if prm.dir_light and dirT is not None:
+        # the directional key light's diffuse term on the room's surfaces:
+        # albedo x colour x max(0, n . -d) x the transmittance toward the
+        # light at the hit x 0.10 (the extension's definition)
+        ndl = max(0.0, -float(nrm @ _dir_of(prm)))
+        col = col + alb * _dir_colour(prm) * (ndl * _sample(dirT, *vuv) * f(0.10))
     ssoot = (1.0 - vis) * float(ss(f(0.18), f(0.82), 1.0 - sh))
     col = lerp(col, F3(0.006, 0.007, 0.008), ssoot * (f(0.38) if surf == 2 else f(0.26)))
     edge = half - abs(hz) if surf == 3 else half - abs(hx)
@@ -1543,7 +1563,7 @@ def test_react_advection_of_nonuniform_fields_matches_primitives():
 
 
 # --------------------------------------------------- raymarch (restatement) ----
-def _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, time):
+def _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, time, dirT=None):
     """renderKernel (:1953-2244) for one pixel, with the build's departures
     (integer-hash jitter of the pixel, step cap 256) and debug modes 0-5.
     Fields, light and noise are sampled at the warped position; the height
@@ -1731,6 +1751,11 @@ def _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, time):
             accum = accum + np.array([1.0, f(0.24), f(0.035)]) * (trans * coal * dt_s)
             accum = accum + scol * (trans * sgain * smask)
             accum = accum + sc_light * (trans * scene_sc * (f(0.18) + damp * f(0.42)))
+            if prm.dir_light and dirT is not None:
+                # single scattering of the directional light: colour x the view
+                # transmittance x the sample's scattering opacity x the
+                # transmittance toward the light x 2.0 (the extension's definition)
+                accum = accum + _dir_colour(prm) * (trans * scalpha * _sample(dirT, u_, v_, w_) * f(2.0))
             fglow = max(fglow, fden * rpow * (f(0.12) + fil * f(0.18) + wfil * f(0.20)))
             socc = sat1(socc + salpha * trans)
             trans *= math.exp(-ext)
@@ -1751,7 +1776,7 @@ def _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, time):
         return np.array([dfuel * f(0.82), dfuel * f(0.48), dfuel * f(0.18), 1.0])
     col = accum
     if trans > f(0.006):
-        col = _room_reference(prm, light, shadow, cam, uv, rd, glow, time) * trans + accum
+        col = _room_reference(prm, light, shadow, cam, uv, rd, glow, time, dirT) * trans + accum
     bloom = min(1.0, max(0.0, fglow * f(0.10))) * (1.0 - min(1.0, max(0.0, socc * f(0.95))))
     col = col + np.array([1.0, f(0.25), f(0.045)]) * (bloom * bloom * f(0.017) * f(prm.reflection_gain))
     return np.append(np.maximum(0.0, col), 1.0)
@@ -1979,4 +2004,80 @@ def test_tonemap_matches_restatement():
             d = (L.or_hash2(x, y, salt) - f(0.5)) / f(255.0)
             r, g, b = (int(min(1.0, max(0.0, c + d)) * f(255.0)) for c in out)
             assert int(got[y, x]) == (0xFF << 24) | (r << 16) | (g << 8) | b, (x, y)
+    o.close()
+
+
+# ------------------------------------------- directional light (restatement) ----
+DIR_LIGHT = dict(dir_light=1, dir_light_x=0.35, dir_light_y=-1.0, dir_light_z=0.25, dir_light_r=1.6, dir_light_g=1.5, dir_light_b=1.35)
+
+
+@pytest.mark.parametrize("sid", [0, 3])
+def test_room_background_with_directional_light_matches_restatement(sid):
+    """The room colour with the directional key light on, against a random
+    transmittance volume: the restatement adds albedo x colour x max(0, n.-d)
+    x T x 0.10 to the pinned room colour, so a slip in the gain, the sign of
+    n.d or the sampling point fails here."""
+    grid = (16, 20, 14)
+    rng = np.random.default_rng(180 + sid)
+    base = dict(BED_PARAMS.get(sid, dict(scene_id=3)))
+    base.update(cinematic_mode=1, noise_seed=19, **DIR_LIGHT)
+    lnx, lny, lnz = firesim.light_dims(*grid)
+    light = rng.uniform(0, 1, (lnz, lny, lnx, 4))
+    light[..., 3] *= 0.6
+    shadow = rng.uniform(0.2, 1, (lnz, lny, lnx))
+    dirT = rng.uniform(0.05, 1, (lnz, lny, lnx))
+    o, prm = scene(grid, frame=(40, 30), camera_distance=1.9, camera_yaw=0.7, camera_pitch=0.2, **base)
+    o.write("light", light)
+    o.write("shadow", shadow)
+    o.write("dir_transmittance", dirT)
+    o.set_time(0.9, 1.7)
+    cam = _camera(prm, 40, 30)
+    out = D3()
+    lit = 0
+    for _ in range(120):
+        rd = rng.normal(size=3)
+        rd /= np.linalg.norm(rd)
+        uv = rng.uniform(0, 1, 2)
+        glow = float(rng.uniform(0, 1))
+        L.or_room_background(o.h, uv[0], uv[1], D3(*rd), glow, out)
+        want = _room_reference(prm, light, shadow, cam, uv, rd, glow, f(1.7), dirT)
+        assert list(out) == pytest.approx(list(want), rel=1e-10, abs=1e-15), rd
+        lit += np.any(np.abs(want - _room_reference(prm, light, shadow, cam, uv, rd, glow, f(1.7))) > 1e-6)
+    assert lit > 30, "the key light must reach many of the surfaces hit"
+    o.close()
+
+
+@pytest.mark.parametrize("sid", [1, 3])
+def test_render_pixels_with_directional_light_match_restatement(sid):
+    """The raymarch with the directional light: the restatement adds colour x
+    trans x the sample's scattering opacity x T x 2.0 at every active sample
+    (and the room's diffuse term behind), on the light pass's own
+    transmittance volume."""
+    grid = (18, 22, 16)
+    fw, fh = 24, 18
+    rng = np.random.default_rng(200 + sid)
+    base = dict(BED_PARAMS.get(sid, dict(scene_id=3)))
+    base.update(raymarch_steps=40, ember_count=0, noise_seed=23, camera_distance=2.6, camera_pitch=0.12, reflection_gain=1.4, smoke=1.1,
+                smoke_darkness=1.2, **DIR_LIGHT)
+    o, prm = scene(grid, frame=(fw, fh), **base)
+    F = _smooth_fields(grid, rng)
+    for n, a in F.items():
+        o.write(n, a)
+    o.set_time(0.45, 0.45)
+    o.run_stage(firesim.STAGE_LIGHT)
+    lnx, lny, lnz = firesim.light_dims(*grid)
+    light = o.read("light").reshape(lnz, lny, lnx, 4)
+    shadow = o.read("shadow").reshape(lnz, lny, lnx)
+    dirT = o.read("dir_transmittance").reshape(lnz, lny, lnx)
+    assert dirT.min() < 0.9 and dirT.max() > 0.1
+    hdr = o.render_hdr()
+    cam = _camera(prm, fw, fh)
+    changed = 0
+    for y in range(2, fh, 3):
+        for x in range(1, fw, 3):
+            want = _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, f(0.45), dirT)
+            assert list(hdr[y, x]) == pytest.approx(list(want), rel=1e-9, abs=1e-13), (sid, x, y)
+            without = _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, f(0.45))
+            changed += np.any(np.abs(want - without) > 1e-6)
+    assert changed > 10, "the key light must change many of the pixels checked"
     o.close()
