@@ -133,6 +133,52 @@ def test_frames_equal_the_per_scene_abi(hip_lib):
 
 
 @pytest.mark.gpu
+def test_reference_interface_frame_matches_the_oracle(hip_lib):
+    """A frame through fireCudaStepAndRender against the FP64 oracle: after
+    three frames through the interface (campfire bed, mouse source, gizmos on by
+    default), the oracle takes the interface's state and clock, the clamped
+    settings, and steps and renders once beside it. Every field of the step is
+    held to relative L2 1e-5 and the BGRA frame (raymarch, tonemap and the
+    gizmo overlay) to the render's FP32 rounding: at most 3 LSB per channel,
+    and 99% of channels within 1 LSB."""
+    import compare
+    import oracle
+
+    assert API.fireCudaInitialize(96, 64, 200, 140), API.last_error()
+    try:
+        frames = _frames(3)
+        a = np.zeros(96 * 64, np.uint32)
+        for s in frames:
+            assert API.step_and_render(a, s, None), API.last_error()
+        api_scene = firesim.Scene.attach(API.fireCudaScene())
+        state = {n: api_scene.read(n) for n in firesim.STATE_FIELDS}
+        t, rt, _ = api_scene.get_time()
+        s = frames[2]
+        assert s.showGizmos == 1 and s.reset == 0
+        o = oracle.OracleScene(*fire_api.reference_grid(200, 140), 96, 64)
+        o.set_params(fire_api.settings_to_params(s))
+        o.reset()
+        for n, arr in state.items():
+            o.write(n, arr.astype(np.float64))
+        o.set_time(t, rt)
+        assert API.step_and_render(a, s, None), API.last_error()
+        o.step(1)
+        errs = {n: compare.rel_l2(api_scene.read(n), o.read(n)) for n in compare.STEP_FIELDS}
+        print("fire_api step vs oracle", errs, flush=True)
+        assert all(v <= 1e-5 for v in errs.values()), errs
+        fo = o.render_bgra8()
+        ca = np.stack([(a.reshape(64, 96) >> k) & 0xFF for k in (0, 8, 16, 24)], -1).astype(int)
+        co = np.stack([(fo >> k) & 0xFF for k in (0, 8, 16, 24)], -1).astype(int)
+        d = np.abs(ca - co)
+        print("fire_api frame vs oracle: max LSB", d.max(), "within 1 LSB", (d <= 1).mean(), flush=True)
+        assert d.max() <= 3 and (d <= 1).mean() >= 0.99
+        api_scene.close()
+        o.close()
+    finally:
+        API.fireCudaShutdown()
+
+
+@pytest.mark.gpu
 def test_reference_smoke_sequence(hip_lib, tmp_path):
     """runCudaSmokeTest: 960x540 frame, 384x240 grid request, 64 frames, BMP of the last one."""
     exe = os.path.join(PKG, "fire_smoke")
