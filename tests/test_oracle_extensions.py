@@ -243,6 +243,47 @@ def test_dir_transmittance_is_beer_lambert_in_a_uniform_medium(direction, ext):
     o.close()
 
 
+@pytest.mark.parametrize("direction", [(0.3, -1.0, 0.2), (-0.5, -0.4, 0.7), (0.0, 0.0, 1.0)])
+def test_dir_transmittance_matches_restatement_in_a_random_medium(direction):
+    """The march from each light-cell centre toward the light, restated from
+    its definition: step ds = half the smallest light cell, samples at
+    ds (k + 1/2) weighted sat(1/2 + (len - s)/ds) over the ray's length len in
+    the volume box, the extinction channel sampled trilinearly at the sample's
+    volume coordinates, T = exp(-3 ds sum). A random medium makes every
+    coordinate offset and scale count (a uniform one cannot see them)."""
+    grid = (24, 20, 18)
+    rng = np.random.default_rng(7)
+    o = _zero_scene(grid, dir_light=1, dir_light_x=direction[0], dir_light_y=direction[1], dir_light_z=direction[2])
+    lnz, lny, lnx, _ = o.shape("light")
+    light = np.zeros((lnz, lny, lnx, 4))
+    light[..., 3] = rng.uniform(0.0, 0.6, (lnz, lny, lnx))
+    o.write("light", light)
+    L.or_dir_transmittance(o.h)
+    got = o.read("dir_transmittance")
+    ext = np.ascontiguousarray(light[..., 3])
+    toward = -_unit32(direction)
+    ds = 0.5 * min(f(2.10) / lnx, f(2.03) / lny, f(1.64) / lnz)
+    for z in range(0, lnz, 2):
+        for y in range(0, lny, 2):
+            for x in range(0, lnx, 3):
+                p0 = np.array([(x + 0.5) / lnx * f(2.10) - f(1.05), (y + 0.5) / lny * f(2.03) + f(0.02), (z + 0.5) / lnz * f(1.64) - f(0.82)])
+                length = _slab_len(p0, toward)
+                tau, k = 0.0, 0
+                while True:
+                    sk = ds * (k + 0.5)
+                    wk = min(1.0, 0.5 + (length - sk) / ds)
+                    if wk <= 0.0:
+                        break
+                    q = p0 + toward * sk
+                    uvw = ((q[0] + f(1.05)) / f(2.10), (q[1] - f(0.02)) / f(2.03), (q[2] + f(0.82)) / f(1.64))
+                    tau += wk * oracle.sample_grid(ext, (lnx - 1, lny - 1, lnz - 1), *uvw)
+                    k += 1
+                # (the FP32 normalization of the direction moves it by up to an ulp)
+                assert got[z, y, x] == pytest.approx(math.exp(-f(3.0) * ds * tau), rel=1e-6), (x, y, z)
+    assert got.min() < 0.5
+    o.close()
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_dir_transmittance_looks_toward_the_light(axis):
     """Smoke in the upper half of one axis, the light arriving from that side:
