@@ -95,6 +95,14 @@ __device__ __forceinline__ void st_at(T* __restrict__ base, Idx i, T v) {
     }
 }
 
+// st_at as a nontemporal store, for a value that only a later launch reads.
+template <class T>
+__device__ __forceinline__ void st_stream(T* __restrict__ base, Idx i, T v) {
+    static_assert(std::is_arithmetic_v<T>, "scalar element types only");
+    FS_GLOBAL char* b = (FS_GLOBAL char*)base;
+    __builtin_nontemporal_store(v, (FS_GLOBAL T*)(b + i * static_cast<Idx>(sizeof(T))));
+}
+
 // Scalar field set. When BF16 is true, heat, fuel and soot are stored as
 // bfloat16 (round to nearest even) and widened to FP32 on load.
 template <bool BF16>

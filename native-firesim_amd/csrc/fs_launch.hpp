@@ -136,8 +136,9 @@ hipError_t launch_batch_light(const BatchTable* table, bool bf16, int grid, hipS
 // Launches one batched step: clock, the six passes. `grid` is the persistent
 // workgroup count (a multiple of 8); `sorLaunches` the pressure launch count.
 hipError_t launch_batch_step(const BatchTable* table, bool bf16, int sorKernel, int sorLaunches, int grid, hipStream_t st);
-// The pressure launches of a batched step alone (autotuning).
-hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sorLaunches, int grid, hipStream_t st);
+// The pressure launches of a batched step alone (autotuning). zeroGuess (tile
+// kernels only): start from p = 0 without reading p.
+hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sorLaunches, int grid, hipStream_t st, bool zeroGuess = false);
 // Persistent grid size for this device (workgroups resident at once, a multiple of 16).
 int batch_grid(int device);
 
@@ -159,13 +160,17 @@ hipError_t launch_advect_velocity(DeviceScene& s, const DevParams& p, hipStream_
 // finalize = false leaves the heat-release sum of the react partials to the
 // next launch_force with finalizeRelease = true (one launch fewer per step).
 hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalize = true);
-hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalizeRelease = false);
-hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st);
+// writeP = false skips the force pass's p = 0 store, for a step whose pressure
+// pass then runs with zeroGuess.
+hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalizeRelease = false, bool writeP = true);
+hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st, bool zeroGuess = false);
 // The pressure pass on bare buffers: p holds the initial guess, p2 is the
 // second buffer the tile kernels ping-pong with (unused by the half-sweep
 // kernel). *resultInP2 tells where the solution ends (an odd launch count).
+// zeroGuess (tile kernels only; the half-sweep kernel is rejected): the
+// initial guess is p = 0 and p is not read.
 hipError_t launch_pressure_buffers(float* p, float* p2, const float* d, int nx, int ny, int nz, float omega, int iterations, int sorKernel,
-                                   hipStream_t st, bool* resultInP2);
+                                   hipStream_t st, bool* resultInP2, bool zeroGuess = false);
 // n floats from a to b (n < 2^30), a plain kernel (no memset or copy node).
 hipError_t launch_copy_f32(const float* a, float* b, size_t n, hipStream_t st);
 hipError_t launch_project(DeviceScene& s, const DevParams& p, hipStream_t st);

@@ -285,14 +285,19 @@ int set_device(fs_scene* s) {
 
 // fused: the step's form, where the force launch also sums the react
 // partials (a pass run alone, or timed alone, finishes its own work).
-int run_stage(fs_scene* s, int stage, const DevParams& p, bool fused = false) {
+// inStep: the pass runs inside a whole step; with a tile SOR kernel the
+// pressure pass then starts from p = 0 without reading p, so the force pass
+// does not store it. A pass run alone keeps the reference's form: force
+// stores p = 0 and pressure starts from p.
+int run_stage(fs_scene* s, int stage, const DevParams& p, bool fused = false, bool inStep = false) {
     hipError_t e = hipSuccess;
     const char* what = "";
+    const bool zeroGuess = inStep && s->ds.sorKernel != 0;
     switch (stage) {
         case FS_STAGE_ADVECT_VELOCITY: e = fs::launch_advect_velocity(s->ds, p, s->stream); what = "advect velocity"; break;
         case FS_STAGE_ADVECT_REACT: e = fs::launch_advect_react(s->ds, p, s->stream, !fused); what = "advect/react"; break;
-        case FS_STAGE_FORCE: e = fs::launch_force(s->ds, p, s->stream, fused); what = "force/boundary/divergence"; break;
-        case FS_STAGE_PRESSURE: e = fs::launch_pressure(s->ds, p, s->stream); what = "pressure SOR"; break;
+        case FS_STAGE_FORCE: e = fs::launch_force(s->ds, p, s->stream, fused, !zeroGuess); what = "force/boundary/divergence"; break;
+        case FS_STAGE_PRESSURE: e = fs::launch_pressure(s->ds, p, s->stream, zeroGuess); what = "pressure SOR"; break;
         case FS_STAGE_PROJECT: e = fs::launch_project(s->ds, p, s->stream); what = "pressure subtract"; break;
         case FS_STAGE_LIGHT: e = fs::launch_light(s->ds, p, s->stream); what = "scene lighting"; break;
         default: return s->fail(FS_ERR_INVALID_ARGUMENT, "unknown stage");
@@ -308,7 +313,7 @@ int enqueue_step(fs_scene* s, const DevParams& p) {
     if (e != hipSuccess) return s->hip_fail("advect velocity", e);
     static_assert(FS_STAGE_FORCE == FS_STAGE_ADVECT_REACT + 1, "force follows advect-react");
     for (int stage = FS_STAGE_ADVECT_VELOCITY + 1; stage < FS_STAGE_COUNT; ++stage)
-        if (int st = run_stage(s, stage, p, true)) return st;
+        if (int st = run_stage(s, stage, p, true, true)) return st;
     return FS_OK;
 }
 
@@ -812,7 +817,7 @@ int fs_step_timed(fs_scene* s, int32_t nsteps, fs_stage_times* out, double* step
         hipError_t err = fs::launch_advance_clock(s->ds, p, s->stream);
         if (err == hipSuccess) err = hipEventRecord(e[0], s->stream);
         for (int stage = 0; err == hipSuccess && status == FS_OK && stage < FS_STAGE_COUNT; ++stage) {
-            status = run_stage(s, stage, p);
+            status = run_stage(s, stage, p, false, true);
             if (status == FS_OK) err = hipEventRecord(e[stage + 1], s->stream);
         }
         if (err != hipSuccess) status = s->hip_fail("hipEventRecord", err);

@@ -654,7 +654,9 @@ __device__ __forceinline__ double face_w(const float* W, const float* cw, const 
 // owns x = x0 + tx, y = y0 + ty): each cell stores its u, v and w faces (and
 // the wall faces past the domain's last column, row and plane), its
 // divergence -(uR - uL + vU - vD + wF - wB) from the unrounded face values, and
-// p = 0 (the divergence kernel :1185-1199). The u and v faces are computed
+// p = 0 (the divergence kernel :1185-1199) unless P is null: a step whose
+// pressure pass starts from zero without reading p (zeroGuess below) skips
+// that store. The u and v faces are computed
 // once: a cell's u face x + 1 is the next lane's u face x and its v face y + 1
 // the v face y of the row above (through LDS); lane 63, the top row of the
 // block and the domain's last column and row compute those faces themselves.
@@ -692,7 +694,7 @@ __device__ __forceinline__ void force_apply_block(float* __restrict__ uo, float*
     const float div = static_cast<float>(uR - uL + vU - vD + wF - wB);
     const Idx idx = sidx(p, x, y, z);
     st_at(D, idx, -div);
-    st_at(P, idx, 0.0f);
+    if (P != nullptr) st_at(P, idx, 0.0f);
 }
 
 __global__ __launch_bounds__(256) void k_force_apply(float* __restrict__ uo, float* __restrict__ vo, float* __restrict__ wo,
@@ -765,11 +767,16 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
 // 22x22 tiles run K = 5: the fewest launches, for small, launch-bound grids.
 // Unroll depth of the sweep loop: 4 for deep tiles (one workgroup per CU,
 // register room to spare), 2 where two workgroups share a CU.
+// The core is written with nontemporal stores: no workgroup of the launch
+// reads it back. At 128^3 that takes a 26x26x13 launch's load-and-store phase
+// from 5.95 to 5.08 us and the 80-iteration solve from 0.584 to 0.567 ms. The
+// projection reading the solved field then runs 3 us slower, but storing the
+// last launch normally costs that launch 4.5 us, so every launch streams.
 constexpr int sor_unroll(int tz) { return tz >= 24 ? 4 : 2; }
 
 template <int T, int K, int TZ, int UNR = sor_unroll(TZ)>
 __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D, float omega, int h0,
-                                         int count, int nx, int ny, int nz, float* __restrict__ sp, uint3 blk) {
+                                         int count, int nx, int ny, int nz, float* __restrict__ sp, uint3 blk, bool zeroIn = false) {
     constexpr int E = T + 2 * K;    // loaded edge in x and y
     constexpr int EZ = TZ + 2 * K;  // loaded depth in z
     constexpr int NT = E * E;  // threads per workgroup
@@ -779,7 +786,11 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
     const int ox = blk.x * T, oy = blk.y * T, oz = blk.z * TZ;
     const int bx = ox - K, by = oy - K, bz = oz - K;  // global coordinate of local 0
     const Idx row = static_cast<Idx>(nx), plane = row * static_cast<Idx>(ny);
-    {
+    if (zeroIn) {
+        // The solve's first launch from p = 0: nothing to load.
+#pragma unroll
+        for (int lz = 0; lz < EZ; ++lz) sp[lz * NT + tid] = 0.0f;
+    } else {
         // Box cells outside the domain take the value of the nearest wall
         // cell (indices clamped per axis). The slot just past a wall then
         // holds that wall cell's value: the reference's missing neighbour
@@ -872,16 +883,16 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
             const int src = (K * E + ty + K) * E + tx + K;
 #pragma unroll
             for (int tz = 0; tz < TZ; ++tz)
-                if (oz + tz < nz) st_at(Pout, static_cast<Idx>(oz + tz) * plane + dst, sp[src + tz * NT]);
+                if (oz + tz < nz) st_stream(Pout, static_cast<Idx>(oz + tz) * plane + dst, sp[src + tz * NT]);
         }
     }
 }
 
 template <int T, int K, int TZ>
 __global__ __launch_bounds__((T + 2 * K) * (T + 2 * K)) void k_sor_tb(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D,
-                                                                     float omega, int h0, int count, int nx, int ny, int nz) {
+                                                                     float omega, int h0, int count, int nx, int ny, int nz, int zeroIn) {
     __shared__ float sp[(T + 2 * K) * (T + 2 * K) * (TZ + 2 * K)];
-    sor_tile<T, K, TZ>(P, Pout, D, omega, h0, count, nx, ny, nz, sp, xcd_block());
+    sor_tile<T, K, TZ>(P, Pout, D, omega, h0, count, nx, ny, nz, sp, xcd_block(), zeroIn != 0);
 }
 
 // ----------------------------------------------------------- K13-K14 ----
@@ -1148,25 +1159,28 @@ __global__ __launch_bounds__(256) void kb_force_contrib(const BatchTable* __rest
     });
 }
 
-__global__ __launch_bounds__(256) void kb_force_apply(const BatchTable* __restrict__ T) {
+// writeP = 0 when the pressure launches start from zero without reading p.
+__global__ __launch_bounds__(256) void kb_force_apply(const BatchTable* __restrict__ T, int writeP) {
     __shared__ double vrow[256];
     batch_blocks(T->cellStart, T->count, [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
         force_apply_block(S.velA[0], S.velA[1], S.velA[2], S.velB[0], S.velB[1], S.velB[2], S.cu, S.cv, S.cw, CurlPsi{S.psi[0], S.psi[1], S.psi[2]},
-                          S.d, S.p0, S.p, block_of(local, S.cellG), vrow);
+                          S.d, writeP ? S.p0 : nullptr, S.p, block_of(local, S.cellG), vrow);
     });
 }
 
-// Pressure launch j: half-sweeps j*K .. j*K+K-1 of every scene that has them.
+// Pressure launch j: half-sweeps j*K .. j*K+K-1 of every scene that has them
+// (launch 0 from p = 0 without reading p when zeroIn is set).
 template <int T_, int K, int TZ>
-__global__ __launch_bounds__((T_ + 2 * K) * (T_ + 2 * K)) void kb_sor_tb(const BatchTable* __restrict__ T, int j) {
+__global__ __launch_bounds__((T_ + 2 * K) * (T_ + 2 * K)) void kb_sor_tb(const BatchTable* __restrict__ T, int j, int zeroIn) {
     __shared__ float sp[(T_ + 2 * K) * (T_ + 2 * K) * (TZ + 2 * K)];
     const int h0 = j * K;
     batch_blocks(T->sorStart, T->sorActive[j], [&](int s, uint32_t local) {
         const BatchScene& S = T->scene[s];
         const float* in = (j & 1) ? S.p1 : S.p0;
         float* out = (j & 1) ? S.p0 : S.p1;
-        sor_tile<T_, K, TZ>(in, out, S.d, S.p.omega, h0, min(K, S.halfSweeps - h0), S.p.nx, S.p.ny, S.p.nz, sp, block_of(local, S.sorG));
+        sor_tile<T_, K, TZ>(in, out, S.d, S.p.omega, h0, min(K, S.halfSweeps - h0), S.p.nx, S.p.ny, S.p.nz, sp, block_of(local, S.sorG),
+                            zeroIn != 0);
         __syncthreads();  // the next tile reuses sp
     });
 }
@@ -1226,13 +1240,14 @@ hipError_t launch_advect_react(DeviceScene& s, const DevParams& p, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalizeRelease) {
+hipError_t launch_force(DeviceScene& s, const DevParams& p, hipStream_t st, bool finalizeRelease, bool writeP) {
     const dim3 g = force_grid(s.nx, s.ny, s.nz), b = cell_block(), bc(64, 4, kForcePlanes);
     const double* part = finalizeRelease ? s.partials : nullptr;
     const size_t np = react_partial_count(s.nx, s.ny, s.nz);
     if (s.bf16) k_force_contrib<true><<<g, bc, 0, st>>>(s.cu, s.cv, s.cw, s.psi[0], s.psi[1], s.psi[2], scal<true>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
     else k_force_contrib<false><<<g, bc, 0, st>>>(s.cu, s.cv, s.cw, s.psi[0], s.psi[1], s.psi[2], scal<false>(s.s), s.u, s.v, s.w, p, part, np, s.acc);
-    k_force_apply<<<cell_grid(s.nx, s.ny, s.nz), b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, s.cu, s.cv, s.cw, CurlPsi{s.psi[0], s.psi[1], s.psi[2]}, s.d, s.p, p);
+    k_force_apply<<<cell_grid(s.nx, s.ny, s.nz), b, 0, st>>>(s.uN, s.vN, s.wN, s.u, s.v, s.w, s.cu, s.cv, s.cw, CurlPsi{s.psi[0], s.psi[1], s.psi[2]}, s.d,
+                                                                     writeP ? s.p : nullptr, p);
     std::swap(s.u, s.uN);
     std::swap(s.v, s.vN);
     std::swap(s.w, s.wN);
@@ -1267,35 +1282,38 @@ int sor_tile_depth(int sorKernel) { return sor_shape(sorKernel).tz; }
 
 // Tiles read the halo of their neighbours, so a launch must not overwrite its
 // own input: launch j reads buffer a (j even) or b (j odd) and writes the other.
+// With zeroGuess the first launch starts from p = 0 and reads nothing from a.
 // Returns the number of launches.
 template <int T, int K, int TZ = T>
-static int launch_sor_tb(float* a, float* b, const float* d, int nx, int ny, int nz, float omega, int iterations, hipStream_t st) {
+static int launch_sor_tb(float* a, float* b, const float* d, int nx, int ny, int nz, float omega, int iterations, hipStream_t st, bool zeroGuess) {
     const dim3 g((nx + T - 1) / T, (ny + T - 1) / T, (nz + TZ - 1) / TZ);
     const int half = 2 * iterations;
     constexpr int E = T + 2 * K;
     int launches = 0;
     for (int h0 = 0; h0 < half; h0 += K, ++launches) {
-        k_sor_tb<T, K, TZ><<<g, E * E, 0, st>>>(a, b, d, omega, h0, min(K, half - h0), nx, ny, nz);
+        k_sor_tb<T, K, TZ><<<g, E * E, 0, st>>>(a, b, d, omega, h0, min(K, half - h0), nx, ny, nz, zeroGuess && h0 == 0);
         std::swap(a, b);
     }
     return launches;
 }
 
 hipError_t launch_pressure_buffers(float* p, float* p2, const float* d, int nx, int ny, int nz, float omega, int iterations, int sorKernel,
-                                   hipStream_t st, bool* resultInP2) {
+                                   hipStream_t st, bool* resultInP2, bool zeroGuess) {
+    if (zeroGuess && sorKernel == 0) return hipErrorInvalidValue;  // the half-sweep kernel updates p in place
+    const bool z = zeroGuess;
     int launches = 0;
     switch (sorKernel) {
-        case 1: launches = launch_sor_tb<16, 4>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 2: launches = launch_sor_tb<24, 4>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 3: launches = launch_sor_tb<24, 4, 8>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 4: launches = launch_sor_tb<24, 4, 16>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 5: launches = launch_sor_tb<24, 4, 28>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 6: launches = launch_sor_tb<24, 4, 10>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 7: launches = launch_sor_tb<24, 4, 12>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 8: launches = launch_sor_tb<24, 4, 6>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 9: launches = launch_sor_tb<26, 3, 13>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 10: launches = launch_sor_tb<26, 3, 9>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
-        case 11: launches = launch_sor_tb<22, 5, 8>(p, p2, d, nx, ny, nz, omega, iterations, st); break;
+        case 1: launches = launch_sor_tb<16, 4>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 2: launches = launch_sor_tb<24, 4>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 3: launches = launch_sor_tb<24, 4, 8>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 4: launches = launch_sor_tb<24, 4, 16>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 5: launches = launch_sor_tb<24, 4, 28>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 6: launches = launch_sor_tb<24, 4, 10>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 7: launches = launch_sor_tb<24, 4, 12>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 8: launches = launch_sor_tb<24, 4, 6>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 9: launches = launch_sor_tb<26, 3, 13>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 10: launches = launch_sor_tb<26, 3, 9>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
+        case 11: launches = launch_sor_tb<22, 5, 8>(p, p2, d, nx, ny, nz, omega, iterations, st, z); break;
         default: {
             const dim3 b = cell_block();
             const dim3 g = sor_grid(0, nx, ny, nz);
@@ -1309,9 +1327,9 @@ hipError_t launch_pressure_buffers(float* p, float* p2, const float* d, int nx, 
     return hipGetLastError();
 }
 
-hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st) {
+hipError_t launch_pressure(DeviceScene& s, const DevParams& p, hipStream_t st, bool zeroGuess) {
     bool inP2 = false;
-    const hipError_t e = launch_pressure_buffers(s.p, s.p2, s.d, s.nx, s.ny, s.nz, p.omega, p.iterations, s.sorKernel, st, &inP2);
+    const hipError_t e = launch_pressure_buffers(s.p, s.p2, s.d, s.nx, s.ny, s.nz, p.omega, p.iterations, s.sorKernel, st, &inP2, zeroGuess);
     if (inP2) std::swap(s.p, s.p2);
     return e;
 }
@@ -1373,14 +1391,14 @@ hipError_t launch_metrics(DeviceScene& s, const DevParams& p, hipStream_t st) {
 }
 
 template <int T, int K, int TZ = T>
-static void launch_batch_sor_tb(const BatchTable* table, int launches, int grid, hipStream_t st) {
+static void launch_batch_sor_tb(const BatchTable* table, int launches, int grid, hipStream_t st, bool zeroGuess) {
     constexpr int E = T + 2 * K, EZ = TZ + 2 * K;
     // as many workgroups as are resident at once: grid / 8 CUs, each holding
     // as many LDS tiles (and E^2-thread workgroups) as fit
     constexpr int byLds = 163840 / (E * E * EZ * 4), byThreads = 2048 / (E * E);
     constexpr int perCu = byLds < byThreads ? byLds : byThreads;
     const int wgs = (grid / 8) * (perCu > 0 ? perCu : 1);
-    for (int j = 0; j < launches; ++j) kb_sor_tb<T, K, TZ><<<wgs, E * E, 0, st>>>(table, j);
+    for (int j = 0; j < launches; ++j) kb_sor_tb<T, K, TZ><<<wgs, E * E, 0, st>>>(table, j, zeroGuess && j == 0);
 }
 
 hipError_t launch_batch_step(const BatchTable* table, bool bf16, int sorKernel, int sorLaunches, int grid, hipStream_t st) {
@@ -1393,8 +1411,11 @@ hipError_t launch_batch_step(const BatchTable* table, bool bf16, int sorKernel, 
     kb_finalize_release<<<kMaxBatch, 256, 0, st>>>(table);
     if (bf16) kb_force_contrib<true><<<grid, b, 0, st>>>(table);
     else kb_force_contrib<false><<<grid, b, 0, st>>>(table);
-    kb_force_apply<<<grid, b, 0, st>>>(table);
-    hipError_t e = launch_batch_pressure(table, sorKernel, sorLaunches, grid, st);
+    // With a tile kernel the pressure launches start from p = 0 without
+    // reading p, so the force pass does not store it.
+    const bool zeroGuess = sorKernel != 0;
+    kb_force_apply<<<grid, b, 0, st>>>(table, zeroGuess ? 0 : 1);
+    hipError_t e = launch_batch_pressure(table, sorKernel, sorLaunches, grid, st, zeroGuess);
     if (e != hipSuccess) return e;
     kb_project<<<grid, b, 0, st>>>(table);
     e = hipGetLastError();
@@ -1402,20 +1423,22 @@ hipError_t launch_batch_step(const BatchTable* table, bool bf16, int sorKernel, 
     return launch_batch_light(table, bf16, grid, st);
 }
 
-hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sorLaunches, int grid, hipStream_t st) {
+hipError_t launch_batch_pressure(const BatchTable* table, int sorKernel, int sorLaunches, int grid, hipStream_t st, bool zeroGuess) {
     if (grid < 16 || grid % 16 != 0 || sorLaunches < 0 || sorLaunches > kMaxSorLaunches) return hipErrorInvalidValue;
+    if (zeroGuess && sorKernel == 0) return hipErrorInvalidValue;  // the half-sweep kernel updates p in place
+    const bool z = zeroGuess;
     switch (sorKernel) {
-        case 1: launch_batch_sor_tb<16, 4>(table, sorLaunches, grid, st); break;
-        case 2: launch_batch_sor_tb<24, 4>(table, sorLaunches, grid, st); break;
-        case 3: launch_batch_sor_tb<24, 4, 8>(table, sorLaunches, grid, st); break;
-        case 4: launch_batch_sor_tb<24, 4, 16>(table, sorLaunches, grid, st); break;
-        case 5: launch_batch_sor_tb<24, 4, 28>(table, sorLaunches, grid, st); break;
-        case 6: launch_batch_sor_tb<24, 4, 10>(table, sorLaunches, grid, st); break;
-        case 7: launch_batch_sor_tb<24, 4, 12>(table, sorLaunches, grid, st); break;
-        case 8: launch_batch_sor_tb<24, 4, 6>(table, sorLaunches, grid, st); break;
-        case 9: launch_batch_sor_tb<26, 3, 13>(table, sorLaunches, grid, st); break;
-        case 10: launch_batch_sor_tb<26, 3, 9>(table, sorLaunches, grid, st); break;
-        case 11: launch_batch_sor_tb<22, 5, 8>(table, sorLaunches, grid, st); break;
+        case 1: launch_batch_sor_tb<16, 4>(table, sorLaunches, grid, st, z); break;
+        case 2: launch_batch_sor_tb<24, 4>(table, sorLaunches, grid, st, z); break;
+        case 3: launch_batch_sor_tb<24, 4, 8>(table, sorLaunches, grid, st, z); break;
+        case 4: launch_batch_sor_tb<24, 4, 16>(table, sorLaunches, grid, st, z); break;
+        case 5: launch_batch_sor_tb<24, 4, 28>(table, sorLaunches, grid, st, z); break;
+        case 6: launch_batch_sor_tb<24, 4, 10>(table, sorLaunches, grid, st, z); break;
+        case 7: launch_batch_sor_tb<24, 4, 12>(table, sorLaunches, grid, st, z); break;
+        case 8: launch_batch_sor_tb<24, 4, 6>(table, sorLaunches, grid, st, z); break;
+        case 9: launch_batch_sor_tb<26, 3, 13>(table, sorLaunches, grid, st, z); break;
+        case 10: launch_batch_sor_tb<26, 3, 9>(table, sorLaunches, grid, st, z); break;
+        case 11: launch_batch_sor_tb<22, 5, 8>(table, sorLaunches, grid, st, z); break;
         default:
             for (int j = 0; j < sorLaunches; ++j) kb_sor_half<<<grid, cell_block(), 0, st>>>(table, j);
     }
