@@ -767,14 +767,16 @@ __global__ __launch_bounds__(256) void k_sor_half(float* __restrict__ P, const f
 // 22x22 tiles run K = 5: the fewest launches, for small, launch-bound grids.
 // Unroll depth of the sweep loop: 4 for deep tiles (one workgroup per CU,
 // register room to spare), 2 where two workgroups share a CU.
-// The core is written with nontemporal stores: no workgroup of the launch
-// reads it back. At 128^3 that takes a 26x26x13 launch's load-and-store phase
-// from 5.95 to 5.08 us and the 80-iteration solve from 0.584 to 0.567 ms. The
-// projection reading the solved field then runs 3 us slower, but storing the
-// last launch normally costs that launch 4.5 us, so every launch streams.
+// With STREAM set (the one-scene kernel), the core is written with
+// nontemporal stores: no workgroup of the launch reads it back. At 128^3 that
+// takes a 26x26x13 launch's load-and-store phase from 5.95 to 5.08 us and the
+// 80-iteration solve from 0.584 to 0.567 ms. The projection reading the solved
+// field then runs 3 us slower, but storing the last launch normally costs that
+// launch 4.5 us, so every launch streams. The batched kernel stores normally:
+// streaming cost the served request stream 2.4% of its rate.
 constexpr int sor_unroll(int tz) { return tz >= 24 ? 4 : 2; }
 
-template <int T, int K, int TZ, int UNR = sor_unroll(TZ)>
+template <int T, int K, int TZ, bool STREAM, int UNR = sor_unroll(TZ)>
 __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D, float omega, int h0,
                                          int count, int nx, int ny, int nz, float* __restrict__ sp, uint3 blk, bool zeroIn = false) {
     constexpr int E = T + 2 * K;    // loaded edge in x and y
@@ -882,8 +884,12 @@ __device__ __forceinline__ void sor_tile(const float* __restrict__ P, float* __r
             const Idx dst = static_cast<Idx>(gy2) * row + static_cast<Idx>(gx);
             const int src = (K * E + ty + K) * E + tx + K;
 #pragma unroll
-            for (int tz = 0; tz < TZ; ++tz)
-                if (oz + tz < nz) st_stream(Pout, static_cast<Idx>(oz + tz) * plane + dst, sp[src + tz * NT]);
+            for (int tz = 0; tz < TZ; ++tz) {
+                if (oz + tz >= nz) continue;
+                const Idx o = static_cast<Idx>(oz + tz) * plane + dst;
+                if constexpr (STREAM) st_stream(Pout, o, sp[src + tz * NT]);
+                else st_at(Pout, o, sp[src + tz * NT]);
+            }
         }
     }
 }
@@ -892,7 +898,7 @@ template <int T, int K, int TZ>
 __global__ __launch_bounds__((T + 2 * K) * (T + 2 * K)) void k_sor_tb(const float* __restrict__ P, float* __restrict__ Pout, const float* __restrict__ D,
                                                                      float omega, int h0, int count, int nx, int ny, int nz, int zeroIn) {
     __shared__ float sp[(T + 2 * K) * (T + 2 * K) * (TZ + 2 * K)];
-    sor_tile<T, K, TZ>(P, Pout, D, omega, h0, count, nx, ny, nz, sp, xcd_block(), zeroIn != 0);
+    sor_tile<T, K, TZ, true>(P, Pout, D, omega, h0, count, nx, ny, nz, sp, xcd_block(), zeroIn != 0);
 }
 
 // ----------------------------------------------------------- K13-K14 ----
@@ -1179,7 +1185,7 @@ __global__ __launch_bounds__((T_ + 2 * K) * (T_ + 2 * K)) void kb_sor_tb(const B
         const BatchScene& S = T->scene[s];
         const float* in = (j & 1) ? S.p1 : S.p0;
         float* out = (j & 1) ? S.p0 : S.p1;
-        sor_tile<T_, K, TZ>(in, out, S.d, S.p.omega, h0, min(K, S.halfSweeps - h0), S.p.nx, S.p.ny, S.p.nz, sp, block_of(local, S.sorG),
+        sor_tile<T_, K, TZ, false>(in, out, S.d, S.p.omega, h0, min(K, S.halfSweeps - h0), S.p.nx, S.p.ny, S.p.nz, sp, block_of(local, S.sorG),
                             zeroIn != 0);
         __syncthreads();  // the next tile reuses sp
     });
