@@ -243,15 +243,18 @@ def test_dir_transmittance_is_beer_lambert_in_a_uniform_medium(direction, ext):
     o.close()
 
 
-@pytest.mark.parametrize("direction", [(0.3, -1.0, 0.2), (-0.5, -0.4, 0.7), (0.0, 0.0, 1.0)])
-def test_dir_transmittance_matches_restatement_in_a_random_medium(direction):
+# Light grids (18, 20, 16), (24, 20, 16) and (18, 20, 18): the smallest light
+# cell, which sets the step, is along y, x and z in turn. The directions leave
+# the volume box through each of its six faces.
+@pytest.mark.parametrize("grid", [(24, 20, 18), (96, 20, 18), (24, 20, 72)])
+@pytest.mark.parametrize("direction", [(0.3, -1.0, 0.2), (-0.5, -0.4, 0.7), (0.0, 0.0, 1.0), (0.2, 0.3, -1.0), (-0.3, 0.8, 0.4)])
+def test_dir_transmittance_matches_restatement_in_a_random_medium(direction, grid):
     """The march from each light-cell centre toward the light, restated from
     its definition: step ds = half the smallest light cell, samples at
     ds (k + 1/2) weighted sat(1/2 + (len - s)/ds) over the ray's length len in
     the volume box, the extinction channel sampled trilinearly at the sample's
     volume coordinates, T = exp(-3 ds sum). A random medium makes every
     coordinate offset and scale count (a uniform one cannot see them)."""
-    grid = (24, 20, 18)
     rng = np.random.default_rng(7)
     o = _zero_scene(grid, dir_light=1, dir_light_x=direction[0], dir_light_y=direction[1], dir_light_z=direction[2])
     lnz, lny, lnx, _ = o.shape("light")
@@ -452,7 +455,9 @@ def test_gizmos_match_reference_restatement(scene_id, active, yaw):
     apply the restated drawGizmos, pack with truncation; equal to the oracle's
     gizmo frame except where a channel sits within roundoff of a byte edge."""
     fw, fh = 240, 180
-    kw = dict(scene_id=scene_id, burner_center_count=1 if scene_id == 2 else 0, camera_yaw=yaw, camera_pitch=0.15, camera_distance=2.4)
+    # (a raised emitter, so the source ring's height term counts)
+    kw = dict(scene_id=scene_id, burner_center_count=1 if scene_id == 2 else 0, camera_yaw=yaw, camera_pitch=0.15, camera_distance=2.4,
+              emitter_height_norm=0.3)
     rng = np.random.default_rng(scene_id)
     hdr = np.concatenate([rng.uniform(0, 2, (fh, fw, 3)) ** 2, np.ones((fh, fw, 1))], -1)
     o = _zero_scene((6, 6, 6), frame=(fw, fh), show_gizmos=0, active_gizmo=active, **kw)
