@@ -1565,7 +1565,7 @@ def test_react_advection_of_nonuniform_fields_matches_primitives():
 # --------------------------------------------------- raymarch (restatement) ----
 def _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, time, dirT=None):
     """renderKernel (:1953-2244) for one pixel, with the build's departures
-    (integer-hash jitter of the pixel, step cap 256) and debug modes 0-5.
+    (integer-hash jitter of the pixel, step cap 256) and debug modes 0-6.
     Fields, light and noise are sampled at the warped position; the height
     gates use the unwarped fv, as the reference does."""
     sid = prm.scene_id
@@ -1585,7 +1585,7 @@ def _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, time, dirT
     rd = rd / math.sqrt(max(f(0.000001), float(rd @ rd)))
     glow = min(1.0, max(0.0, S("heat", f(0.50), f(0.06), f(0.50)) * f(0.42)))
     accum, trans, fglow, socc = np.zeros(3), 1.0, 0.0, 0.0
-    dflame = dod = dtemp = dfuel = 0.0
+    dflame = dod = dtemp = dfuel = dvel = 0.0
     eye = cam["eye"]
     # intersectBox (:1867-1907)
     bmin, bmax = np.array([f(-1.05), f(0.02), f(-0.82)]), np.array([f(1.05), f(2.05), f(0.82)])
@@ -1633,6 +1633,11 @@ def _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, time, dirT
                 i += 1
                 continue
             dfuel = max(dfuel, fade * min(1.0, max(0.0, fuel * f(0.18) + char * f(0.22) + pyro * f(0.30))))
+            if prm.render_debug_mode == 6:  # debugVelocity (:2054): the staggered velocity at the warped sample
+                vx = oracle.sample_grid(F["u"], (nx, ny - 1, nz - 1), u_, v_, w_)
+                vy = oracle.sample_grid(F["v"], (nx - 1, ny, nz - 1), u_, v_, w_)
+                vz = oracle.sample_grid(F["w"], (nx - 1, ny - 1, nz), u_, v_, w_)
+                dvel = max(dvel, min(1.0, max(0.0, math.sqrt(vx * vx + vy * vy + vz * vz) * f(0.22))))
             cx, cy, cz = f(1.0 / nx), f(1.0 / ny), f(1.0 / nz)
             gx = S("heat", u_ + cx, v_, w_) - S("heat", u_ - cx, v_, w_)
             gy = S("heat", u_, v_ + cy, w_) - S("heat", u_, v_ - cy, w_)
@@ -1773,7 +1778,9 @@ def _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, time, dirT
             return np.array([trans, trans, trans, 1.0])
         if mode == 4:
             return np.append(heatcol(dtemp), 1.0)
-        return np.array([dfuel * f(0.82), dfuel * f(0.48), dfuel * f(0.18), 1.0])
+        if mode == 5:
+            return np.array([dfuel * f(0.82), dfuel * f(0.48), dfuel * f(0.18), 1.0])
+        return np.array([dvel * f(0.20), dvel * f(0.72), dvel, 1.0])
     col = accum
     if trans > f(0.006):
         col = _room_reference(prm, light, shadow, cam, uv, rd, glow, time, dirT) * trans + accum
@@ -1791,19 +1798,39 @@ def _debug_heat_color(value):
 
 # (2, 0, narrow band): a burner band so narrow that the port jet's width floor
 # (0.012 of the height) takes over from 0.68 of the band.
+# (0, 0, _heat): heat weak enough that the glow sampled at (0.5, 0.06, 0.5) for
+# the room stays below saturation, so its sample point and gain show.
+# (3, 0, _scale): every scalar so thin that the coarse density along the rays
+# crosses the empty-space skip threshold (0.0015) and the activity threshold.
+# (3, 6): the velocity view over random staggered velocities.
 @pytest.mark.parametrize("sid,mode,extra", [(0, 0, {}), (1, 0, {}), (2, 0, {}), (2, 0, dict(emitter_height_band_norm=0.008)), (3, 0, {}), (3, 1, {}),
-                                            (3, 2, {}), (3, 4, {}), (0, 5, {})])
+                                            (3, 2, {}), (3, 4, {}), (0, 5, {}), (3, 6, {}), (0, 0, dict(_heat=0.12)),
+                                            (3, 0, dict(_scale=0.0012))])
 def test_render_pixels_match_restatement(sid, mode, extra):
     """Selected pixels of a small frame over a fire blob (flame, soot, coal glow,
     room behind), every scene's flame model and the debug views."""
     grid = (18, 22, 16)
     fw, fh = 24, 18
+    extra = dict(extra)
+    heat_k, all_k = extra.pop("_heat", None), extra.pop("_scale", None)
     rng = np.random.default_rng(100 + sid)
     base = dict(BED_PARAMS.get(sid, dict(scene_id=3)))
     base.update(render_debug_mode=mode, raymarch_steps=40, ember_count=0, noise_seed=23, camera_distance=2.6, camera_pitch=0.12,
                 reflection_gain=1.4, smoke=1.1, smoke_darkness=1.2, **extra)
     o, prm = scene(grid, frame=(fw, fh), **base)
     F = _smooth_fields(grid, rng)
+    nx, ny, nz = grid
+    if heat_k is not None:
+        F["heat"] = F["heat"] * heat_k
+        glow = min(1.0, max(0.0, oracle.sample_grid(F["heat"], (nx - 1, ny - 1, nz - 1), f(0.50), f(0.06), f(0.50)) * f(0.42)))
+        assert 0.05 < glow < 0.95, glow
+    if all_k is not None:
+        F = {n: a * all_k if n != "oxygen" else a for n, a in F.items()}
+        assert 0.002 < F["heat"].max() < 0.02
+    if mode == 6:
+        F["u"] = rng.uniform(-2, 2, (nz, ny, nx + 1))
+        F["v"] = rng.uniform(-1, 4, (nz, ny + 1, nx))
+        F["w"] = rng.uniform(-2, 2, (nz + 1, ny, nx))
     for n, a in F.items():
         o.write(n, a)
     o.set_time(0.45, 0.45)
@@ -1819,8 +1846,10 @@ def test_render_pixels_match_restatement(sid, mode, extra):
         want = _render_pixel_reference(prm, F, light, shadow, cam, x, y, fw, fh, f(0.45))
         assert list(hdr[y, x]) == pytest.approx(list(want), rel=1e-12, abs=1e-15), (sid, mode, x, y)
         emissive += want[0] > 0.05
-    if mode == 0:
+    if mode == 0 and heat_k is None and all_k is None:
         assert emissive >= 3, "the blob must emit in some of the pixels checked"
+    if mode == 6:
+        assert emissive >= 3, "the velocity view must show in some of the pixels checked"
     o.close()
 
 
@@ -1970,9 +1999,13 @@ def _embers_reference(prm, F, cam, fw, fh, time):
 
 
 # amp 0.25: fields weak enough that many sparks sit on the ramp of their
-# material gate instead of at its top.
-@pytest.mark.parametrize("sid,amp", [(0, 1.0), (1, 1.0), (2, 1.0), (3, 1.0), (0, 0.25), (1, 0.25)])
-def test_embers_match_restatement(sid, amp):
+# material gate instead of at its top. _heat 0.05: heat alone so weak that the
+# glow sampled at (0.5, 0.06, 0.5) lies on the ramp of the spark gate (0.02 to
+# 0.30) while char and pyrolysis keep the material gates open. The burner with
+# its smallest radius (0.035) starts sparks inside the 0.04 floor.
+@pytest.mark.parametrize("sid,amp,extra", [(0, 1.0, {}), (1, 1.0, {}), (2, 1.0, {}), (3, 1.0, {}), (0, 0.25, {}), (1, 0.25, {}),
+                                           (0, 1.0, dict(_heat=0.05)), (3, 1.0, dict(_heat=0.05)), (2, 1.0, dict(emitter_radius=0.035))])
+def test_embers_match_restatement(sid, amp, extra):
     """The ember increments of the HDR frame (frame with sparks minus frame
     without) equal the restated splat of every spark."""
     grid = (18, 22, 16)
@@ -1980,13 +2013,19 @@ def test_embers_match_restatement(sid, amp):
     fw, fh = (320, 240) if sid == 2 else (48, 36)
     rng = np.random.default_rng(120 + sid)
     base = dict(BED_PARAMS.get(sid, dict(scene_id=3)))
-    base.update(raymarch_steps=24, ember_count=0, noise_seed=31, camera_distance=2.4, wind_x=0.3)
+    extra = dict(extra)
+    heat_k = extra.pop("_heat", None)
+    base.update(raymarch_steps=24, ember_count=0, noise_seed=31, camera_distance=2.4, wind_x=0.3, **extra)
     o, prm = scene(grid, frame=(fw, fh), **base)
     F = _smooth_fields(grid, rng)
     F["heat"] = F["heat"] * 1.5
     for n in ("heat", "char", "pyrolysis"):
         F[n] = F[n] * amp
     nx, ny, nz = grid
+    if heat_k is not None:
+        F["heat"] = F["heat"] * heat_k
+        glow = min(1.0, max(0.0, oracle.sample_grid(F["heat"], (nx - 1, ny - 1, nz - 1), f(0.50), f(0.06), f(0.50)) * f(0.42)))
+        assert 0.02 < glow < 0.30, glow
     F["u"] = rng.uniform(-1, 1, (nz, ny, nx + 1))
     F["v"] = rng.uniform(-1, 2, (nz, ny + 1, nx))
     F["w"] = rng.uniform(-1, 1, (nz + 1, ny, nx))
