@@ -253,7 +253,7 @@ def plain_serial_step_ms(grid, params, emitters, steps):
     return dt * 1e3
 
 
-def bench_batched(seed: int, steps: int, count: int):
+def bench_batched(seed: int, steps: int, count: int, use_graph: bool = True):
     """Uneven batch: `count` scenes of the batch shapes in one persistent
     launch per pass, FP32 and BF16 storage, against the same scenes stepped
     one at a time on the plain FP32 path. (Per-field error of batched FP32
@@ -267,8 +267,12 @@ def bench_batched(seed: int, steps: int, count: int):
         with firesim.Batch() as batch:
             for sc in scenes:
                 batch.add(sc)
+            if not use_graph:
+                batch.set_tuning(use_graph=False)
             batch.step(2)
             tuned = batch.autotune(reps=1)
+            if not use_graph:
+                batch.set_tuning(sor_kernel=tuned["chosen"], use_graph=False)
             batch.step(3)  # the tuned kernel's step graph is captured here, not in the timed steps
             batch.synchronize()
             t0 = time.perf_counter()
@@ -277,7 +281,7 @@ def bench_batched(seed: int, steps: int, count: int):
         for sc in scenes:
             sc.close()
         result[mode] = {"voxel_steps_per_s": cells * steps / wall, "ms_per_step": wall * 1e3 / steps, "step_ms": pcts(ms),
-                        "sor_kernel": tuned["chosen"]}
+                        "sor_kernel": tuned["chosen"], "step_path": "captured HIP graph" if use_graph else "direct launches"}
     # the same scenes one at a time on the plain FP32 path (3 timed steps each)
     plain_ms = sum(plain_serial_step_ms(sp["grid"], sp["params"], sp["emitters"], 3) for sp in specs)
     result["plain_fp32_serial"] = {"voxel_steps_per_s": cells / (plain_ms * 1e-3), "ms_per_step": plain_ms}
@@ -377,6 +381,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-scene uneven batch")
     ap.add_argument("--batched-scenes", type=int, default=48)
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--batched-no-graph", action="store_true", help="direct launches instead of captured batch step graphs")
     ap.add_argument("--only", choices=["single", "batched", "served"], default=None, help="run one section only")
     ap.add_argument("--from-reset", action="store_true",
                     help="time the steps right after --warmup untimed steps from reset (short profiled runs), "
@@ -391,7 +396,7 @@ def main():
         calibrate_copy()
         return
     if args.only == "batched":
-        print(json.dumps({"batched": bench_batched(args.seed, args.batched_steps, args.batched_scenes)}))
+        print(json.dumps({"batched": bench_batched(args.seed, args.batched_steps, args.batched_scenes, not args.batched_no_graph)}))
         return
     if args.only == "served":
         storage = firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16
@@ -478,7 +483,7 @@ def main():
     progress("per-pass timing and render done")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.config, args.seed, args.cpu_budget, start_state, start)
     progress("cpu baseline done")
-    batched = None if args.no_batched else bench_batched(args.seed, args.batched_steps, args.batched_scenes)
+    batched = None if args.no_batched else bench_batched(args.seed, args.batched_steps, args.batched_scenes, not args.batched_no_graph)
     progress("batched done")
     served = None if args.no_served else bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb,
                                                       firesim.STORAGE_FP32 if args.served_fp32 else firesim.STORAGE_BF16,
