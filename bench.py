@@ -19,6 +19,8 @@ Printed: one JSON line with the contract keys plus
   stages        per-pass times and algorithmic-bandwidth fractions,
   step_latency  p50/p99 of the per-step GPU times in the timed region,
   render        one HDR raymarch + tonemap frame after the timed steps,
+  tall          the 256x384x256 scene (8 emitters, SOR 120): its last 10
+                steps, and its pressure kernel's rate against HBM peak,
   served        a Poisson stream of requests (cubic grids 32..192, lognormal
                 step counts, BF16 heat/fuel/soot, a 256x256 frame every 10
                 steps) through the continuous-batching server: p50/p99 step
@@ -289,6 +291,37 @@ def bench_batched(seed: int, steps: int, count: int, use_graph: bool = True):
     return result
 
 
+def bench_tall(seed: int, steps: int = 10):
+    """The tall single scene (256x384x256, 8 emitters, SOR 120): the last
+    `steps` of its 100-step run on the captured-graph step path, then the same
+    number of steps with HIP events between passes for the pressure kernel's
+    algorithmic rate (the configuration the reference's bandwidth is quoted on)."""
+    cfg, params, emitters = workload("c3", seed)
+    nx, ny, nz = cfg["grid"]
+    cells = nx * ny * nz
+    scene = firesim.Scene(nx, ny, nz, 64, 64)
+    scene.set_params(params)
+    scene.set_emitters(emitters)
+    scene.reset()
+    scene.step(cfg["steps"] - steps)
+    scene.synchronize()
+    scene.autotune(reps=1)
+    scene.step_latency(0)  # captures the chosen kernel's step graphs outside the timed steps
+    t0 = time.perf_counter()
+    scene.step_latency(steps)
+    wall = time.perf_counter() - t0
+    times, _ = scene.step_timed(steps)
+    ms = times.stage_ms[firesim.STAGE_PRESSURE] / steps
+    launches = times.pressure_launches
+    gbs = PASS_BYTES["pressure"] * params.sor_iterations * cells / (ms * 1e-3) / 1e9
+    out = {"workload": cfg["desc"], "grid": list(cfg["grid"]), "sor_iterations": params.sor_iterations, "timed_steps": f"{cfg['steps'] - steps + 1}-{cfg['steps']}",
+           "voxel_steps_per_s": cells * steps / wall, "ms_per_step": wall * 1e3 / steps, "sor_kernel": firesim.SOR_KERNELS[scene.sor_kernel()],
+           "pressure": {"ms": round(ms, 4), "launches": launches, "launch_ms": round(ms / launches, 5), "alg_GBps": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+    scene.close()
+    return out
+
+
 def _serve(reqs, budget_gb: float, storage: int, use_graph: bool, max_round_voxels: int = 0, sor_kernel: str = "tb_t24_z28_k4") -> dict:
     """One pass of the request stream through fs_server; its statistics."""
     # A fixed SOR tile (default 24x24x28, the fastest batched tile on this
@@ -379,6 +412,7 @@ def main():
     ap.add_argument("--served-target-ms", type=float, default=10.0, help="step latency target of the capped served pass")
     ap.add_argument("--served-sor-kernel", default="tb_t24_z28_k4", help="fixed SOR tile of the served stream (a name of firesim.SOR_KERNELS)")
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-scene uneven batch")
+    ap.add_argument("--no-tall", action="store_true", help="skip the 256x384x256 scene's pressure rate")
     ap.add_argument("--batched-scenes", type=int, default=48)
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-no-graph", action="store_true", help="direct launches instead of captured batch step graphs")
@@ -404,7 +438,7 @@ def main():
                                                  not args.served_no_graph, args.served_target_ms, args.served_sor_kernel)}))
         return
     if args.only == "single":
-        args.no_batched = args.no_served = True
+        args.no_batched = args.no_served = args.no_tall = True
 
     cfg, params, emitters = workload(args.config, args.seed)
     nx, ny, nz = cfg["grid"]
@@ -483,6 +517,8 @@ def main():
     progress("per-pass timing and render done")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.config, args.seed, args.cpu_budget, start_state, start)
     progress("cpu baseline done")
+    tall = None if args.no_tall or args.config != "c2" else bench_tall(args.seed)
+    progress("tall scene done")
     batched = None if args.no_batched else bench_batched(args.seed, args.batched_steps, args.batched_scenes, not args.batched_no_graph)
     progress("batched done")
     served = None if args.no_served else bench_served(args.seed, args.served_requests, args.served_rate, args.served_budget_gb,
@@ -516,6 +552,7 @@ def main():
         "render": render,
         "sanity": {"invalid_cells": metrics["invalid_cells"], "heat_release_total": metrics["heat_release_total"],
                    "divergence_after_l2": metrics["divergence_after_l2"]},
+        "tall": tall,
         "served": served,
         "batched": batched,
     }
