@@ -9,7 +9,8 @@ GPU's developed state (the FP64 oracle would need hours to reach it):
   * 256x384x256 with 8 seeded emitters, SOR 120: 99 steps, then one step of
     each side; the 1280x720 frame;
   * a served BF16 request stream with grids up to 192^3: the largest request's
-    scene after its last served step, one more step on each side.
+    scene after its last served step, one more step on each side;
+  * the 128^3 workload run to 1000 steps: finite and inside the clamps.
 
 Bounds as in the other parity tests: relative L2 <= 1e-5 per field for FP32
 steps, 1e-3 for HDR frames; BF16 storage with the bounds of the batched
@@ -145,3 +146,31 @@ def test_config_served_bf16_up_to_192(hip_lib):
         o.close()
     finally:
         s.close()
+
+
+def test_config_128_long_run_stays_bounded(hip_lib):
+    """The 128^3 workload run to five times its 200 steps (1000 steps on the
+    captured-graph path). Every 100 steps: no cell is NaN or infinite, the
+    scalars stay inside advect-react's clamps (heat 8.8, fuel 5.5, soot 7.4,
+    pyrolysis 3.6, progress 3.4, turbulence energy 4.2), the projection reduces
+    the divergence, and the total heat released is finite and growing; at the
+    end every field is finite."""
+    cfg, params, emitters = workloads.single_scene("c2", 2)
+    s = firesim.Scene(*cfg["grid"], 64, 64)
+    s.set_params(params)
+    s.set_emitters(emitters)
+    s.reset()
+    released = 0.0
+    for _ in range(10):
+        s.step(100)
+        m = s.metrics()
+        assert m["invalid_cells"] == 0, m["frame_index"]
+        for key, cap in (("max_heat", 8.8), ("max_fuel", 5.5), ("max_soot", 7.4), ("max_pyrolysis", 3.6), ("max_progress", 3.4),
+                         ("max_turbulence_energy", 4.2)):
+            assert 0.0 <= m[key] <= np.float32(cap), (key, m[key])
+        assert m["divergence_after_l2"] < m["divergence_before_l2"]
+        assert np.isfinite(m["heat_release_total"]) and m["heat_release_total"] > released
+        released = m["heat_release_total"]
+    for name in firesim.STATE_FIELDS:
+        assert np.all(np.isfinite(s.read(name))), name
+    s.close()
