@@ -1453,6 +1453,7 @@ ROOM_FEATURES = [
     (-0.92, 1.0, 2.55), (0.92, 0.5, 2.55), (0.0, 1.72, 2.55), (0.92, 1.72, 2.55), (0.5, 0.28, 2.55),  # recess (:1838-1843)
     (0.92, 0.29, 2.55), (-0.92, 0.3, 2.55), (-0.92, 1.71, 2.55),
     (-1.2, 2.18, 0.5), (0.0, 2.18, -0.7), (-1.5, 2.18, 0.92), (1.72, 2.18, 0.8),  # ceiling panels, fixtures (:1820-1827)
+    (-1.2, 2.18, -2.0), (-1.2, 2.18, 2.0), (0.0, 2.18, -2.0), (0.0, 2.18, 2.0),  # ends of the ceiling panel lines
     (1.08, 0.0, 0.2), (0.3, 0.0, 0.58), (0.96, 0.0, -0.1), (0.2, 0.0, 0.47),  # tray body and inner edges (:1770-1772)
     (0.47, 0.0, -0.03), (0.05, 0.0, 0.1884), (0.05, 0.0, -0.03), (0.0, 0.0, 0.75),  # burner ring, inner ring, pilot, plate (:1754-1760)
 ]
