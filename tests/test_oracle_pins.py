@@ -547,6 +547,32 @@ def test_light_build_and_propagate_match_restatement():
     o.close()
 
 
+def test_light_build_on_a_warm_even_field_matches_restatement():
+    """A hot field with almost no heat gradient (heat 2 with a 2% ripple): the
+    reaction front is then carried by fuel x oxygen and progress alone and sits
+    on the ramps of its smoothsteps, and of the flame sheet's gate on it, in
+    cells hot enough to emit, which a fire blob (steep edges, saturated front
+    wherever it is hot) never shows."""
+    grid = (10, 12, 8)
+    rng = np.random.default_rng(33)
+    o, prm = scene(grid, noise_seed=9, reflection_gain=1.3)
+    shape = (grid[2], grid[1], grid[0])
+    level = dict(heat=2.0, fuel=1.0, oxygen=0.5, soot=0.2, char=0.1, ash=0.05, pyrolysis=0.3, progress=0.9, turbulence=0.3, soot_optics=0.4)
+    F = {n: v * (1.0 + 0.02 * rng.uniform(-1, 1, shape)) for n, v in level.items()}
+    for n, a in F.items():
+        o.write(n, a)
+    o.set_time(0.8, 0.8)
+    L.or_light_build(o.h)
+    lnx, lny, lnz = firesim.light_dims(*grid)
+    want_l, want_s = _light_build_reference(F, lnx, lny, lnz, prm, float(np.float32(0.8)))
+    got_l = o.read("light").reshape(lnz, lny, lnx, 4)
+    got_s = o.read("shadow").reshape(lnz, lny, lnx)
+    assert want_l[..., :3].max() > 0.01, "the fields must light some cells"
+    assert np.allclose(got_l, want_l, rtol=1e-11, atol=1e-14)
+    assert np.allclose(got_s, want_s, rtol=1e-11, atol=1e-14)
+    o.close()
+
+
 def test_light_propagate_uniform_closed_form():
     """A uniform light field (rgb c, extinction e) with uniform shadow s: the
     occlusion-weighted neighbour mean is c, so next = c decay (0.66 + 0.34 s)
